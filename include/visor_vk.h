@@ -1,0 +1,1388 @@
+/*
+ * visor_vk.h - the Vulkan 1.0 subset this ICD implements, declared from the public Vulkan 1.0
+ * specification (struct layouts, enum values and entry-point signatures are fixed by that ABI).
+ *
+ * The container has no Vulkan SDK, so the driver and its tests declare exactly what they use.
+ * Layouts are the standard 64-bit ones: dispatchable handles are pointers to objects whose first
+ * word is the loader magic (ICD loader interface, reference 3rdparty/vk_icd.h:54-64); non-dispatchable
+ * handles are pointers to opaque structs.
+ */
+#ifndef VISOR_VK_H
+#define VISOR_VK_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define VKAPI_ATTR
+#define VKAPI_CALL
+#define VKAPI_PTR
+
+#define VK_MAKE_VERSION(major, minor, patch) (((major) << 22) | ((minor) << 12) | (patch))
+#define VK_TRUE 1u
+#define VK_FALSE 0u
+#define VK_NULL_HANDLE 0
+#define VK_WHOLE_SIZE (~0ULL)
+#define VK_MAX_EXTENSION_NAME_SIZE 256
+#define VK_MAX_PHYSICAL_DEVICE_NAME_SIZE 256
+#define VK_UUID_SIZE 16
+#define VK_MAX_MEMORY_TYPES 32
+#define VK_MAX_MEMORY_HEAPS 16
+
+#define ICD_LOADER_MAGIC 0x01CDC0DE
+#define VISOR_LOADER_ICD_INTERFACE_VERSION 4
+
+typedef uint32_t VkFlags;
+typedef uint32_t VkBool32;
+typedef uint64_t VkDeviceSize;
+typedef uint32_t VkSampleMask;
+
+#define VK_DEFINE_HANDLE(object) typedef struct object##_T *object;
+#define VK_DEFINE_NON_DISPATCHABLE_HANDLE(object) typedef struct object##_T *object;
+
+VK_DEFINE_HANDLE(VkInstance)
+VK_DEFINE_HANDLE(VkPhysicalDevice)
+VK_DEFINE_HANDLE(VkDevice)
+VK_DEFINE_HANDLE(VkQueue)
+VK_DEFINE_HANDLE(VkCommandBuffer)
+VK_DEFINE_NON_DISPATCHABLE_HANDLE(VkSemaphore)
+VK_DEFINE_NON_DISPATCHABLE_HANDLE(VkFence)
+VK_DEFINE_NON_DISPATCHABLE_HANDLE(VkDeviceMemory)
+VK_DEFINE_NON_DISPATCHABLE_HANDLE(VkBuffer)
+VK_DEFINE_NON_DISPATCHABLE_HANDLE(VkImage)
+VK_DEFINE_NON_DISPATCHABLE_HANDLE(VkEvent)
+VK_DEFINE_NON_DISPATCHABLE_HANDLE(VkQueryPool)
+VK_DEFINE_NON_DISPATCHABLE_HANDLE(VkBufferView)
+VK_DEFINE_NON_DISPATCHABLE_HANDLE(VkImageView)
+VK_DEFINE_NON_DISPATCHABLE_HANDLE(VkShaderModule)
+VK_DEFINE_NON_DISPATCHABLE_HANDLE(VkPipelineCache)
+VK_DEFINE_NON_DISPATCHABLE_HANDLE(VkPipelineLayout)
+VK_DEFINE_NON_DISPATCHABLE_HANDLE(VkRenderPass)
+VK_DEFINE_NON_DISPATCHABLE_HANDLE(VkPipeline)
+VK_DEFINE_NON_DISPATCHABLE_HANDLE(VkDescriptorSetLayout)
+VK_DEFINE_NON_DISPATCHABLE_HANDLE(VkSampler)
+VK_DEFINE_NON_DISPATCHABLE_HANDLE(VkDescriptorPool)
+VK_DEFINE_NON_DISPATCHABLE_HANDLE(VkDescriptorSet)
+VK_DEFINE_NON_DISPATCHABLE_HANDLE(VkFramebuffer)
+VK_DEFINE_NON_DISPATCHABLE_HANDLE(VkCommandPool)
+VK_DEFINE_NON_DISPATCHABLE_HANDLE(VkSurfaceKHR)
+VK_DEFINE_NON_DISPATCHABLE_HANDLE(VkSwapchainKHR)
+
+typedef enum VkResult {
+  VK_SUCCESS = 0,
+  VK_NOT_READY = 1,
+  VK_TIMEOUT = 2,
+  VK_EVENT_SET = 3,
+  VK_EVENT_RESET = 4,
+  VK_INCOMPLETE = 5,
+  VK_ERROR_OUT_OF_HOST_MEMORY = -1,
+  VK_ERROR_OUT_OF_DEVICE_MEMORY = -2,
+  VK_ERROR_INITIALIZATION_FAILED = -3,
+  VK_ERROR_DEVICE_LOST = -4,
+  VK_ERROR_MEMORY_MAP_FAILED = -5,
+  VK_ERROR_LAYER_NOT_PRESENT = -6,
+  VK_ERROR_EXTENSION_NOT_PRESENT = -7,
+  VK_ERROR_FEATURE_NOT_PRESENT = -8,
+  VK_ERROR_INCOMPATIBLE_DRIVER = -9,
+  VK_ERROR_TOO_MANY_OBJECTS = -10,
+  VK_ERROR_FORMAT_NOT_SUPPORTED = -11,
+  VK_RESULT_MAX_ENUM = 0x7FFFFFFF
+} VkResult;
+
+typedef enum VkStructureType {
+  VK_STRUCTURE_TYPE_APPLICATION_INFO = 0,
+  VK_STRUCTURE_TYPE_INSTANCE_CREATE_INFO = 1,
+  VK_STRUCTURE_TYPE_DEVICE_QUEUE_CREATE_INFO = 2,
+  VK_STRUCTURE_TYPE_DEVICE_CREATE_INFO = 3,
+  VK_STRUCTURE_TYPE_SUBMIT_INFO = 4,
+  VK_STRUCTURE_TYPE_MEMORY_ALLOCATE_INFO = 5,
+  VK_STRUCTURE_TYPE_MAPPED_MEMORY_RANGE = 6,
+  VK_STRUCTURE_TYPE_FENCE_CREATE_INFO = 8,
+  VK_STRUCTURE_TYPE_SEMAPHORE_CREATE_INFO = 9,
+  VK_STRUCTURE_TYPE_BUFFER_CREATE_INFO = 12,
+  VK_STRUCTURE_TYPE_IMAGE_CREATE_INFO = 14,
+  VK_STRUCTURE_TYPE_IMAGE_VIEW_CREATE_INFO = 15,
+  VK_STRUCTURE_TYPE_SHADER_MODULE_CREATE_INFO = 16,
+  VK_STRUCTURE_TYPE_PIPELINE_CACHE_CREATE_INFO = 17,
+  VK_STRUCTURE_TYPE_PIPELINE_SHADER_STAGE_CREATE_INFO = 18,
+  VK_STRUCTURE_TYPE_PIPELINE_VERTEX_INPUT_STATE_CREATE_INFO = 19,
+  VK_STRUCTURE_TYPE_PIPELINE_INPUT_ASSEMBLY_STATE_CREATE_INFO = 20,
+  VK_STRUCTURE_TYPE_PIPELINE_TESSELLATION_STATE_CREATE_INFO = 21,
+  VK_STRUCTURE_TYPE_PIPELINE_VIEWPORT_STATE_CREATE_INFO = 22,
+  VK_STRUCTURE_TYPE_PIPELINE_RASTERIZATION_STATE_CREATE_INFO = 23,
+  VK_STRUCTURE_TYPE_PIPELINE_MULTISAMPLE_STATE_CREATE_INFO = 24,
+  VK_STRUCTURE_TYPE_PIPELINE_DEPTH_STENCIL_STATE_CREATE_INFO = 25,
+  VK_STRUCTURE_TYPE_PIPELINE_COLOR_BLEND_STATE_CREATE_INFO = 26,
+  VK_STRUCTURE_TYPE_PIPELINE_DYNAMIC_STATE_CREATE_INFO = 27,
+  VK_STRUCTURE_TYPE_GRAPHICS_PIPELINE_CREATE_INFO = 28,
+  VK_STRUCTURE_TYPE_COMPUTE_PIPELINE_CREATE_INFO = 29,
+  VK_STRUCTURE_TYPE_PIPELINE_LAYOUT_CREATE_INFO = 30,
+  VK_STRUCTURE_TYPE_SAMPLER_CREATE_INFO = 31,
+  VK_STRUCTURE_TYPE_DESCRIPTOR_SET_LAYOUT_CREATE_INFO = 32,
+  VK_STRUCTURE_TYPE_DESCRIPTOR_POOL_CREATE_INFO = 33,
+  VK_STRUCTURE_TYPE_DESCRIPTOR_SET_ALLOCATE_INFO = 34,
+  VK_STRUCTURE_TYPE_WRITE_DESCRIPTOR_SET = 35,
+  VK_STRUCTURE_TYPE_COPY_DESCRIPTOR_SET = 36,
+  VK_STRUCTURE_TYPE_FRAMEBUFFER_CREATE_INFO = 37,
+  VK_STRUCTURE_TYPE_RENDER_PASS_CREATE_INFO = 38,
+  VK_STRUCTURE_TYPE_COMMAND_POOL_CREATE_INFO = 39,
+  VK_STRUCTURE_TYPE_COMMAND_BUFFER_ALLOCATE_INFO = 40,
+  VK_STRUCTURE_TYPE_COMMAND_BUFFER_INHERITANCE_INFO = 41,
+  VK_STRUCTURE_TYPE_COMMAND_BUFFER_BEGIN_INFO = 42,
+  VK_STRUCTURE_TYPE_RENDER_PASS_BEGIN_INFO = 43,
+  VK_STRUCTURE_TYPE_BUFFER_MEMORY_BARRIER = 44,
+  VK_STRUCTURE_TYPE_IMAGE_MEMORY_BARRIER = 45,
+  VK_STRUCTURE_TYPE_MEMORY_BARRIER = 46,
+  VK_STRUCTURE_TYPE_SWAPCHAIN_CREATE_INFO_KHR = 1000001000,
+  VK_STRUCTURE_TYPE_PRESENT_INFO_KHR = 1000001001,
+  VK_STRUCTURE_TYPE_MAX_ENUM = 0x7FFFFFFF
+} VkStructureType;
+
+typedef enum VkFormat {
+  VK_FORMAT_UNDEFINED = 0,
+  VK_FORMAT_R8_UNORM = 9,
+  VK_FORMAT_R8G8B8A8_UNORM = 37,
+  VK_FORMAT_R8G8B8A8_SRGB = 43,
+  VK_FORMAT_B8G8R8A8_UNORM = 44,
+  VK_FORMAT_B8G8R8A8_SRGB = 50,
+  VK_FORMAT_R16G16B16A16_SFLOAT = 97,
+  VK_FORMAT_R32_UINT = 98,
+  VK_FORMAT_R32_SINT = 99,
+  VK_FORMAT_R32_SFLOAT = 100,
+  VK_FORMAT_R32G32_UINT = 101,
+  VK_FORMAT_R32G32_SINT = 102,
+  VK_FORMAT_R32G32_SFLOAT = 103,
+  VK_FORMAT_R32G32B32_UINT = 104,
+  VK_FORMAT_R32G32B32_SINT = 105,
+  VK_FORMAT_R32G32B32_SFLOAT = 106,
+  VK_FORMAT_R32G32B32A32_UINT = 107,
+  VK_FORMAT_R32G32B32A32_SINT = 108,
+  VK_FORMAT_R32G32B32A32_SFLOAT = 109,
+  VK_FORMAT_D32_SFLOAT = 126,
+  VK_FORMAT_D32_SFLOAT_S8_UINT = 130,
+  VK_FORMAT_BC2_UNORM_BLOCK = 135,
+  VK_FORMAT_BC3_UNORM_BLOCK = 137,
+  VK_FORMAT_MAX_ENUM = 0x7FFFFFFF
+} VkFormat;
+
+typedef enum VkPhysicalDeviceType {
+  VK_PHYSICAL_DEVICE_TYPE_OTHER = 0,
+  VK_PHYSICAL_DEVICE_TYPE_INTEGRATED_GPU = 1,
+  VK_PHYSICAL_DEVICE_TYPE_DISCRETE_GPU = 2,
+  VK_PHYSICAL_DEVICE_TYPE_VIRTUAL_GPU = 3,
+  VK_PHYSICAL_DEVICE_TYPE_CPU = 4,
+  VK_PHYSICAL_DEVICE_TYPE_MAX_ENUM = 0x7FFFFFFF
+} VkPhysicalDeviceType;
+
+typedef enum VkImageType {
+  VK_IMAGE_TYPE_1D = 0,
+  VK_IMAGE_TYPE_2D = 1,
+  VK_IMAGE_TYPE_3D = 2,
+  VK_IMAGE_TYPE_MAX_ENUM = 0x7FFFFFFF
+} VkImageType;
+
+typedef enum VkImageViewType {
+  VK_IMAGE_VIEW_TYPE_1D = 0,
+  VK_IMAGE_VIEW_TYPE_2D = 1,
+  VK_IMAGE_VIEW_TYPE_3D = 2,
+  VK_IMAGE_VIEW_TYPE_CUBE = 3,
+  VK_IMAGE_VIEW_TYPE_1D_ARRAY = 4,
+  VK_IMAGE_VIEW_TYPE_2D_ARRAY = 5,
+  VK_IMAGE_VIEW_TYPE_CUBE_ARRAY = 6,
+  VK_IMAGE_VIEW_TYPE_MAX_ENUM = 0x7FFFFFFF
+} VkImageViewType;
+
+typedef enum VkPrimitiveTopology {
+  VK_PRIMITIVE_TOPOLOGY_POINT_LIST = 0,
+  VK_PRIMITIVE_TOPOLOGY_LINE_LIST = 1,
+  VK_PRIMITIVE_TOPOLOGY_LINE_STRIP = 2,
+  VK_PRIMITIVE_TOPOLOGY_TRIANGLE_LIST = 3,
+  VK_PRIMITIVE_TOPOLOGY_TRIANGLE_STRIP = 4,
+  VK_PRIMITIVE_TOPOLOGY_TRIANGLE_FAN = 5,
+  VK_PRIMITIVE_TOPOLOGY_MAX_ENUM = 0x7FFFFFFF
+} VkPrimitiveTopology;
+
+typedef enum VkFrontFace {
+  VK_FRONT_FACE_COUNTER_CLOCKWISE = 0,
+  VK_FRONT_FACE_CLOCKWISE = 1,
+  VK_FRONT_FACE_MAX_ENUM = 0x7FFFFFFF
+} VkFrontFace;
+
+typedef enum VkCullModeFlagBits {
+  VK_CULL_MODE_NONE = 0,
+  VK_CULL_MODE_FRONT_BIT = 1,
+  VK_CULL_MODE_BACK_BIT = 2,
+  VK_CULL_MODE_FRONT_AND_BACK = 3,
+  VK_CULL_MODE_FLAG_BITS_MAX_ENUM = 0x7FFFFFFF
+} VkCullModeFlagBits;
+typedef VkFlags VkCullModeFlags;
+
+typedef enum VkPolygonMode {
+  VK_POLYGON_MODE_FILL = 0,
+  VK_POLYGON_MODE_LINE = 1,
+  VK_POLYGON_MODE_POINT = 2,
+  VK_POLYGON_MODE_MAX_ENUM = 0x7FFFFFFF
+} VkPolygonMode;
+
+typedef enum VkCompareOp {
+  VK_COMPARE_OP_NEVER = 0,
+  VK_COMPARE_OP_LESS = 1,
+  VK_COMPARE_OP_EQUAL = 2,
+  VK_COMPARE_OP_LESS_OR_EQUAL = 3,
+  VK_COMPARE_OP_GREATER = 4,
+  VK_COMPARE_OP_NOT_EQUAL = 5,
+  VK_COMPARE_OP_GREATER_OR_EQUAL = 6,
+  VK_COMPARE_OP_ALWAYS = 7,
+  VK_COMPARE_OP_MAX_ENUM = 0x7FFFFFFF
+} VkCompareOp;
+
+typedef enum VkStencilOp {
+  VK_STENCIL_OP_KEEP = 0,
+  VK_STENCIL_OP_MAX_ENUM = 0x7FFFFFFF
+} VkStencilOp;
+
+typedef enum VkLogicOp {
+  VK_LOGIC_OP_CLEAR = 0,
+  VK_LOGIC_OP_COPY = 3,
+  VK_LOGIC_OP_MAX_ENUM = 0x7FFFFFFF
+} VkLogicOp;
+
+typedef enum VkBlendFactor {
+  VK_BLEND_FACTOR_ZERO = 0,
+  VK_BLEND_FACTOR_ONE = 1,
+  VK_BLEND_FACTOR_SRC_COLOR = 2,
+  VK_BLEND_FACTOR_ONE_MINUS_SRC_COLOR = 3,
+  VK_BLEND_FACTOR_DST_COLOR = 4,
+  VK_BLEND_FACTOR_ONE_MINUS_DST_COLOR = 5,
+  VK_BLEND_FACTOR_SRC_ALPHA = 6,
+  VK_BLEND_FACTOR_ONE_MINUS_SRC_ALPHA = 7,
+  VK_BLEND_FACTOR_DST_ALPHA = 8,
+  VK_BLEND_FACTOR_ONE_MINUS_DST_ALPHA = 9,
+  VK_BLEND_FACTOR_CONSTANT_COLOR = 10,
+  VK_BLEND_FACTOR_ONE_MINUS_CONSTANT_COLOR = 11,
+  VK_BLEND_FACTOR_CONSTANT_ALPHA = 12,
+  VK_BLEND_FACTOR_ONE_MINUS_CONSTANT_ALPHA = 13,
+  VK_BLEND_FACTOR_SRC_ALPHA_SATURATE = 14,
+  VK_BLEND_FACTOR_SRC1_COLOR = 15,
+  VK_BLEND_FACTOR_ONE_MINUS_SRC1_COLOR = 16,
+  VK_BLEND_FACTOR_SRC1_ALPHA = 17,
+  VK_BLEND_FACTOR_ONE_MINUS_SRC1_ALPHA = 18,
+  VK_BLEND_FACTOR_MAX_ENUM = 0x7FFFFFFF
+} VkBlendFactor;
+
+typedef enum VkBlendOp {
+  VK_BLEND_OP_ADD = 0,
+  VK_BLEND_OP_SUBTRACT = 1,
+  VK_BLEND_OP_REVERSE_SUBTRACT = 2,
+  VK_BLEND_OP_MIN = 3,
+  VK_BLEND_OP_MAX = 4,
+  VK_BLEND_OP_MAX_ENUM = 0x7FFFFFFF
+} VkBlendOp;
+
+typedef enum VkDynamicState {
+  VK_DYNAMIC_STATE_VIEWPORT = 0,
+  VK_DYNAMIC_STATE_SCISSOR = 1,
+  VK_DYNAMIC_STATE_MAX_ENUM = 0x7FFFFFFF
+} VkDynamicState;
+
+typedef enum VkFilter {
+  VK_FILTER_NEAREST = 0,
+  VK_FILTER_LINEAR = 1,
+  VK_FILTER_MAX_ENUM = 0x7FFFFFFF
+} VkFilter;
+
+typedef enum VkSamplerMipmapMode {
+  VK_SAMPLER_MIPMAP_MODE_NEAREST = 0,
+  VK_SAMPLER_MIPMAP_MODE_LINEAR = 1,
+  VK_SAMPLER_MIPMAP_MODE_MAX_ENUM = 0x7FFFFFFF
+} VkSamplerMipmapMode;
+
+typedef enum VkSamplerAddressMode {
+  VK_SAMPLER_ADDRESS_MODE_REPEAT = 0,
+  VK_SAMPLER_ADDRESS_MODE_MIRRORED_REPEAT = 1,
+  VK_SAMPLER_ADDRESS_MODE_CLAMP_TO_EDGE = 2,
+  VK_SAMPLER_ADDRESS_MODE_CLAMP_TO_BORDER = 3,
+  VK_SAMPLER_ADDRESS_MODE_MAX_ENUM = 0x7FFFFFFF
+} VkSamplerAddressMode;
+
+typedef enum VkBorderColor {
+  VK_BORDER_COLOR_FLOAT_TRANSPARENT_BLACK = 0,
+  VK_BORDER_COLOR_MAX_ENUM = 0x7FFFFFFF
+} VkBorderColor;
+
+typedef enum VkDescriptorType {
+  VK_DESCRIPTOR_TYPE_SAMPLER = 0,
+  VK_DESCRIPTOR_TYPE_COMBINED_IMAGE_SAMPLER = 1,
+  VK_DESCRIPTOR_TYPE_SAMPLED_IMAGE = 2,
+  VK_DESCRIPTOR_TYPE_STORAGE_IMAGE = 3,
+  VK_DESCRIPTOR_TYPE_UNIFORM_TEXEL_BUFFER = 4,
+  VK_DESCRIPTOR_TYPE_STORAGE_TEXEL_BUFFER = 5,
+  VK_DESCRIPTOR_TYPE_UNIFORM_BUFFER = 6,
+  VK_DESCRIPTOR_TYPE_STORAGE_BUFFER = 7,
+  VK_DESCRIPTOR_TYPE_UNIFORM_BUFFER_DYNAMIC = 8,
+  VK_DESCRIPTOR_TYPE_STORAGE_BUFFER_DYNAMIC = 9,
+  VK_DESCRIPTOR_TYPE_INPUT_ATTACHMENT = 10,
+  VK_DESCRIPTOR_TYPE_MAX_ENUM = 0x7FFFFFFF
+} VkDescriptorType;
+
+typedef enum VkAttachmentLoadOp {
+  VK_ATTACHMENT_LOAD_OP_LOAD = 0,
+  VK_ATTACHMENT_LOAD_OP_CLEAR = 1,
+  VK_ATTACHMENT_LOAD_OP_DONT_CARE = 2,
+  VK_ATTACHMENT_LOAD_OP_MAX_ENUM = 0x7FFFFFFF
+} VkAttachmentLoadOp;
+
+typedef enum VkAttachmentStoreOp {
+  VK_ATTACHMENT_STORE_OP_STORE = 0,
+  VK_ATTACHMENT_STORE_OP_DONT_CARE = 1,
+  VK_ATTACHMENT_STORE_OP_MAX_ENUM = 0x7FFFFFFF
+} VkAttachmentStoreOp;
+
+typedef enum VkPipelineBindPoint {
+  VK_PIPELINE_BIND_POINT_GRAPHICS = 0,
+  VK_PIPELINE_BIND_POINT_COMPUTE = 1,
+  VK_PIPELINE_BIND_POINT_MAX_ENUM = 0x7FFFFFFF
+} VkPipelineBindPoint;
+
+typedef enum VkCommandBufferLevel {
+  VK_COMMAND_BUFFER_LEVEL_PRIMARY = 0,
+  VK_COMMAND_BUFFER_LEVEL_SECONDARY = 1,
+  VK_COMMAND_BUFFER_LEVEL_MAX_ENUM = 0x7FFFFFFF
+} VkCommandBufferLevel;
+
+typedef enum VkIndexType {
+  VK_INDEX_TYPE_UINT16 = 0,
+  VK_INDEX_TYPE_UINT32 = 1,
+  VK_INDEX_TYPE_MAX_ENUM = 0x7FFFFFFF
+} VkIndexType;
+
+typedef enum VkSubpassContents {
+  VK_SUBPASS_CONTENTS_INLINE = 0,
+  VK_SUBPASS_CONTENTS_MAX_ENUM = 0x7FFFFFFF
+} VkSubpassContents;
+
+typedef enum VkVertexInputRate {
+  VK_VERTEX_INPUT_RATE_VERTEX = 0,
+  VK_VERTEX_INPUT_RATE_INSTANCE = 1,
+  VK_VERTEX_INPUT_RATE_MAX_ENUM = 0x7FFFFFFF
+} VkVertexInputRate;
+
+typedef enum VkSharingMode {
+  VK_SHARING_MODE_EXCLUSIVE = 0,
+  VK_SHARING_MODE_CONCURRENT = 1,
+  VK_SHARING_MODE_MAX_ENUM = 0x7FFFFFFF
+} VkSharingMode;
+
+typedef enum VkImageTiling {
+  VK_IMAGE_TILING_OPTIMAL = 0,
+  VK_IMAGE_TILING_LINEAR = 1,
+  VK_IMAGE_TILING_MAX_ENUM = 0x7FFFFFFF
+} VkImageTiling;
+
+typedef enum VkImageLayout {
+  VK_IMAGE_LAYOUT_UNDEFINED = 0,
+  VK_IMAGE_LAYOUT_GENERAL = 1,
+  VK_IMAGE_LAYOUT_COLOR_ATTACHMENT_OPTIMAL = 2,
+  VK_IMAGE_LAYOUT_DEPTH_STENCIL_ATTACHMENT_OPTIMAL = 3,
+  VK_IMAGE_LAYOUT_SHADER_READ_ONLY_OPTIMAL = 5,
+  VK_IMAGE_LAYOUT_TRANSFER_DST_OPTIMAL = 7,
+  VK_IMAGE_LAYOUT_PREINITIALIZED = 8,
+  VK_IMAGE_LAYOUT_PRESENT_SRC_KHR = 1000001002,
+  VK_IMAGE_LAYOUT_MAX_ENUM = 0x7FFFFFFF
+} VkImageLayout;
+
+typedef enum VkComponentSwizzle {
+  VK_COMPONENT_SWIZZLE_IDENTITY = 0,
+  VK_COMPONENT_SWIZZLE_MAX_ENUM = 0x7FFFFFFF
+} VkComponentSwizzle;
+
+typedef enum VkColorSpaceKHR {
+  VK_COLOR_SPACE_SRGB_NONLINEAR_KHR = 0,
+  VK_COLOR_SPACE_MAX_ENUM_KHR = 0x7FFFFFFF
+} VkColorSpaceKHR;
+
+typedef enum VkPresentModeKHR {
+  VK_PRESENT_MODE_IMMEDIATE_KHR = 0,
+  VK_PRESENT_MODE_MAILBOX_KHR = 1,
+  VK_PRESENT_MODE_FIFO_KHR = 2,
+  VK_PRESENT_MODE_FIFO_RELAXED_KHR = 3,
+  VK_PRESENT_MODE_MAX_ENUM_KHR = 0x7FFFFFFF
+} VkPresentModeKHR;
+
+typedef enum VkSurfaceTransformFlagBitsKHR {
+  VK_SURFACE_TRANSFORM_IDENTITY_BIT_KHR = 1,
+  VK_SURFACE_TRANSFORM_FLAG_BITS_MAX_ENUM_KHR = 0x7FFFFFFF
+} VkSurfaceTransformFlagBitsKHR;
+
+typedef enum VkCompositeAlphaFlagBitsKHR {
+  VK_COMPOSITE_ALPHA_OPAQUE_BIT_KHR = 1,
+  VK_COMPOSITE_ALPHA_FLAG_BITS_MAX_ENUM_KHR = 0x7FFFFFFF
+} VkCompositeAlphaFlagBitsKHR;
+
+typedef enum VkSampleCountFlagBits {
+  VK_SAMPLE_COUNT_1_BIT = 1,
+  VK_SAMPLE_COUNT_FLAG_BITS_MAX_ENUM = 0x7FFFFFFF
+} VkSampleCountFlagBits;
+
+typedef enum VkShaderStageFlagBits {
+  VK_SHADER_STAGE_VERTEX_BIT = 0x00000001,
+  VK_SHADER_STAGE_FRAGMENT_BIT = 0x00000010,
+  VK_SHADER_STAGE_COMPUTE_BIT = 0x00000020,
+  VK_SHADER_STAGE_ALL_GRAPHICS = 0x0000001F,
+  VK_SHADER_STAGE_FLAG_BITS_MAX_ENUM = 0x7FFFFFFF
+} VkShaderStageFlagBits;
+
+enum {
+  VK_QUEUE_GRAPHICS_BIT = 1,
+  VK_QUEUE_COMPUTE_BIT = 2,
+  VK_QUEUE_TRANSFER_BIT = 4,
+  VK_MEMORY_PROPERTY_DEVICE_LOCAL_BIT = 1,
+  VK_MEMORY_PROPERTY_HOST_VISIBLE_BIT = 2,
+  VK_MEMORY_PROPERTY_HOST_COHERENT_BIT = 4,
+  VK_MEMORY_PROPERTY_HOST_CACHED_BIT = 8,
+  VK_MEMORY_HEAP_DEVICE_LOCAL_BIT = 1,
+  VK_IMAGE_ASPECT_COLOR_BIT = 1,
+  VK_IMAGE_ASPECT_DEPTH_BIT = 2,
+  VK_FENCE_CREATE_SIGNALED_BIT = 1,
+  VK_FORMAT_FEATURE_SAMPLED_IMAGE_BIT = 0x1,
+  VK_FORMAT_FEATURE_UNIFORM_TEXEL_BUFFER_BIT = 0x8,
+  VK_FORMAT_FEATURE_VERTEX_BUFFER_BIT = 0x40,
+  VK_FORMAT_FEATURE_COLOR_ATTACHMENT_BIT = 0x80,
+  VK_FORMAT_FEATURE_COLOR_ATTACHMENT_BLEND_BIT = 0x100,
+  VK_FORMAT_FEATURE_DEPTH_STENCIL_ATTACHMENT_BIT = 0x200,
+  VK_FORMAT_FEATURE_TRANSFER_SRC_BIT_KHR = 0x4000,
+  VK_FORMAT_FEATURE_TRANSFER_DST_BIT_KHR = 0x8000,
+  VK_COLOR_COMPONENT_ALL = 0xF
+};
+
+typedef VkFlags VkInstanceCreateFlags;
+typedef VkFlags VkDeviceCreateFlags;
+typedef VkFlags VkDeviceQueueCreateFlags;
+typedef VkFlags VkPipelineStageFlags;
+typedef VkFlags VkMemoryMapFlags;
+typedef VkFlags VkFenceCreateFlags;
+typedef VkFlags VkSemaphoreCreateFlags;
+typedef VkFlags VkBufferCreateFlags;
+typedef VkFlags VkBufferUsageFlags;
+typedef VkFlags VkImageCreateFlags;
+typedef VkFlags VkImageUsageFlags;
+typedef VkFlags VkImageViewCreateFlags;
+typedef VkFlags VkImageAspectFlags;
+typedef VkFlags VkShaderModuleCreateFlags;
+typedef VkFlags VkPipelineCacheCreateFlags;
+typedef VkFlags VkPipelineCreateFlags;
+typedef VkFlags VkPipelineShaderStageCreateFlags;
+typedef VkFlags VkPipelineVertexInputStateCreateFlags;
+typedef VkFlags VkPipelineInputAssemblyStateCreateFlags;
+typedef VkFlags VkPipelineTessellationStateCreateFlags;
+typedef VkFlags VkPipelineViewportStateCreateFlags;
+typedef VkFlags VkPipelineRasterizationStateCreateFlags;
+typedef VkFlags VkPipelineMultisampleStateCreateFlags;
+typedef VkFlags VkPipelineDepthStencilStateCreateFlags;
+typedef VkFlags VkPipelineColorBlendStateCreateFlags;
+typedef VkFlags VkPipelineDynamicStateCreateFlags;
+typedef VkFlags VkPipelineLayoutCreateFlags;
+typedef VkFlags VkShaderStageFlags;
+typedef VkFlags VkSamplerCreateFlags;
+typedef VkFlags VkDescriptorSetLayoutCreateFlags;
+typedef VkFlags VkDescriptorPoolCreateFlags;
+typedef VkFlags VkFramebufferCreateFlags;
+typedef VkFlags VkRenderPassCreateFlags;
+typedef VkFlags VkAttachmentDescriptionFlags;
+typedef VkFlags VkSubpassDescriptionFlags;
+typedef VkFlags VkAccessFlags;
+typedef VkFlags VkDependencyFlags;
+typedef VkFlags VkCommandPoolCreateFlags;
+typedef VkFlags VkCommandBufferUsageFlags;
+typedef VkFlags VkQueryControlFlags;
+typedef VkFlags VkQueryPipelineStatisticFlags;
+typedef VkFlags VkColorComponentFlags;
+typedef VkFlags VkSampleCountFlags;
+typedef VkFlags VkQueueFlags;
+typedef VkFlags VkMemoryPropertyFlags;
+typedef VkFlags VkMemoryHeapFlags;
+typedef VkFlags VkFormatFeatureFlags;
+typedef VkFlags VkSurfaceTransformFlagsKHR;
+typedef VkFlags VkCompositeAlphaFlagsKHR;
+typedef VkFlags VkSwapchainCreateFlagsKHR;
+
+typedef void(VKAPI_PTR *PFN_vkVoidFunction)(void);
+
+typedef struct VkOffset2D { int32_t x, y; } VkOffset2D;
+typedef struct VkOffset3D { int32_t x, y, z; } VkOffset3D;
+typedef struct VkExtent2D { uint32_t width, height; } VkExtent2D;
+typedef struct VkExtent3D { uint32_t width, height, depth; } VkExtent3D;
+typedef struct VkRect2D { VkOffset2D offset; VkExtent2D extent; } VkRect2D;
+
+typedef struct VkApplicationInfo {
+  VkStructureType sType;
+  const void *pNext;
+  const char *pApplicationName;
+  uint32_t applicationVersion;
+  const char *pEngineName;
+  uint32_t engineVersion;
+  uint32_t apiVersion;
+} VkApplicationInfo;
+
+typedef struct VkInstanceCreateInfo {
+  VkStructureType sType;
+  const void *pNext;
+  VkInstanceCreateFlags flags;
+  const VkApplicationInfo *pApplicationInfo;
+  uint32_t enabledLayerCount;
+  const char *const *ppEnabledLayerNames;
+  uint32_t enabledExtensionCount;
+  const char *const *ppEnabledExtensionNames;
+} VkInstanceCreateInfo;
+
+typedef struct VkAllocationCallbacks VkAllocationCallbacks; /* accepted and ignored */
+
+typedef struct VkPhysicalDeviceFeatures {
+  VkBool32 robustBufferAccess;
+  VkBool32 fullDrawIndexUint32;
+  VkBool32 imageCubeArray;
+  VkBool32 independentBlend;
+  VkBool32 geometryShader;
+  VkBool32 tessellationShader;
+  VkBool32 sampleRateShading;
+  VkBool32 dualSrcBlend;
+  VkBool32 logicOp;
+  VkBool32 multiDrawIndirect;
+  VkBool32 drawIndirectFirstInstance;
+  VkBool32 depthClamp;
+  VkBool32 depthBiasClamp;
+  VkBool32 fillModeNonSolid;
+  VkBool32 depthBounds;
+  VkBool32 wideLines;
+  VkBool32 largePoints;
+  VkBool32 alphaToOne;
+  VkBool32 multiViewport;
+  VkBool32 samplerAnisotropy;
+  VkBool32 textureCompressionETC2;
+  VkBool32 textureCompressionASTC_LDR;
+  VkBool32 textureCompressionBC;
+  VkBool32 remaining[32]; /* occlusionQueryPrecise .. inheritedQueries, all VK_FALSE here */
+} VkPhysicalDeviceFeatures;
+
+typedef struct VkPhysicalDeviceLimits {
+  uint32_t maxImageDimension1D;
+  uint32_t maxImageDimension2D;
+  uint32_t maxImageDimension3D;
+  uint32_t maxImageDimensionCube;
+  uint32_t maxImageArrayLayers;
+  uint32_t maxTexelBufferElements;
+  uint32_t maxUniformBufferRange;
+  uint32_t maxStorageBufferRange;
+  uint32_t maxPushConstantsSize;
+  uint32_t maxMemoryAllocationCount;
+  uint32_t maxSamplerAllocationCount;
+  VkDeviceSize bufferImageGranularity;
+  VkDeviceSize sparseAddressSpaceSize;
+  uint32_t maxBoundDescriptorSets;
+  uint32_t maxPerStageDescriptorSamplers;
+  uint32_t maxPerStageDescriptorUniformBuffers;
+  uint32_t maxPerStageDescriptorStorageBuffers;
+  uint32_t maxPerStageDescriptorSampledImages;
+  uint32_t maxPerStageDescriptorStorageImages;
+  uint32_t maxPerStageDescriptorInputAttachments;
+  uint32_t maxPerStageResources;
+  uint32_t maxDescriptorSetSamplers;
+  uint32_t maxDescriptorSetUniformBuffers;
+  uint32_t maxDescriptorSetUniformBuffersDynamic;
+  uint32_t maxDescriptorSetStorageBuffers;
+  uint32_t maxDescriptorSetStorageBuffersDynamic;
+  uint32_t maxDescriptorSetSampledImages;
+  uint32_t maxDescriptorSetStorageImages;
+  uint32_t maxDescriptorSetInputAttachments;
+  uint32_t maxVertexInputAttributes;
+  uint32_t maxVertexInputBindings;
+  uint32_t maxVertexInputAttributeOffset;
+  uint32_t maxVertexInputBindingStride;
+  uint32_t maxVertexOutputComponents;
+  uint32_t maxTessellationGenerationLevel;
+  uint32_t maxTessellationPatchSize;
+  uint32_t maxTessellationControlPerVertexInputComponents;
+  uint32_t maxTessellationControlPerVertexOutputComponents;
+  uint32_t maxTessellationControlPerPatchOutputComponents;
+  uint32_t maxTessellationControlTotalOutputComponents;
+  uint32_t maxTessellationEvaluationInputComponents;
+  uint32_t maxTessellationEvaluationOutputComponents;
+  uint32_t maxGeometryShaderInvocations;
+  uint32_t maxGeometryInputComponents;
+  uint32_t maxGeometryOutputComponents;
+  uint32_t maxGeometryOutputVertices;
+  uint32_t maxGeometryTotalOutputComponents;
+  uint32_t maxFragmentInputComponents;
+  uint32_t maxFragmentOutputAttachments;
+  uint32_t maxFragmentDualSrcAttachments;
+  uint32_t maxFragmentCombinedOutputResources;
+  uint32_t maxComputeSharedMemorySize;
+  uint32_t maxComputeWorkGroupCount[3];
+  uint32_t maxComputeWorkGroupInvocations;
+  uint32_t maxComputeWorkGroupSize[3];
+  uint32_t subPixelPrecisionBits;
+  uint32_t subTexelPrecisionBits;
+  uint32_t mipmapPrecisionBits;
+  uint32_t maxDrawIndexedIndexValue;
+  uint32_t maxDrawIndirectCount;
+  float maxSamplerLodBias;
+  float maxSamplerAnisotropy;
+  uint32_t maxViewports;
+  uint32_t maxViewportDimensions[2];
+  float viewportBoundsRange[2];
+  uint32_t viewportSubPixelBits;
+  size_t minMemoryMapAlignment;
+  VkDeviceSize minTexelBufferOffsetAlignment;
+  VkDeviceSize minUniformBufferOffsetAlignment;
+  VkDeviceSize minStorageBufferOffsetAlignment;
+  int32_t minTexelOffset;
+  uint32_t maxTexelOffset;
+  int32_t minTexelGatherOffset;
+  uint32_t maxTexelGatherOffset;
+  float minInterpolationOffset;
+  float maxInterpolationOffset;
+  uint32_t subPixelInterpolationOffsetBits;
+  uint32_t maxFramebufferWidth;
+  uint32_t maxFramebufferHeight;
+  uint32_t maxFramebufferLayers;
+  VkSampleCountFlags framebufferColorSampleCounts;
+  VkSampleCountFlags framebufferDepthSampleCounts;
+  VkSampleCountFlags framebufferStencilSampleCounts;
+  VkSampleCountFlags framebufferNoAttachmentsSampleCounts;
+  uint32_t maxColorAttachments;
+  VkSampleCountFlags sampledImageColorSampleCounts;
+  VkSampleCountFlags sampledImageIntegerSampleCounts;
+  VkSampleCountFlags sampledImageDepthSampleCounts;
+  VkSampleCountFlags sampledImageStencilSampleCounts;
+  VkSampleCountFlags storageImageSampleCounts;
+  uint32_t maxSampleMaskWords;
+  VkBool32 timestampComputeAndGraphics;
+  float timestampPeriod;
+  uint32_t maxClipDistances;
+  uint32_t maxCullDistances;
+  uint32_t maxCombinedClipAndCullDistances;
+  uint32_t discreteQueuePriorities;
+  float pointSizeRange[2];
+  float lineWidthRange[2];
+  float pointSizeGranularity;
+  float lineWidthGranularity;
+  VkBool32 strictLines;
+  VkBool32 standardSampleLocations;
+  VkDeviceSize optimalBufferCopyOffsetAlignment;
+  VkDeviceSize optimalBufferCopyRowPitchAlignment;
+  VkDeviceSize nonCoherentAtomSize;
+} VkPhysicalDeviceLimits;
+
+typedef struct VkPhysicalDeviceSparseProperties {
+  VkBool32 residencyStandard2DBlockShape;
+  VkBool32 residencyStandard2DMultisampleBlockShape;
+  VkBool32 residencyStandard3DBlockShape;
+  VkBool32 residencyAlignedMipSize;
+  VkBool32 residencyNonResidentStrict;
+} VkPhysicalDeviceSparseProperties;
+
+typedef struct VkPhysicalDeviceProperties {
+  uint32_t apiVersion;
+  uint32_t driverVersion;
+  uint32_t vendorID;
+  uint32_t deviceID;
+  VkPhysicalDeviceType deviceType;
+  char deviceName[VK_MAX_PHYSICAL_DEVICE_NAME_SIZE];
+  uint8_t pipelineCacheUUID[VK_UUID_SIZE];
+  VkPhysicalDeviceLimits limits;
+  VkPhysicalDeviceSparseProperties sparseProperties;
+} VkPhysicalDeviceProperties;
+
+typedef struct VkQueueFamilyProperties {
+  VkQueueFlags queueFlags;
+  uint32_t queueCount;
+  uint32_t timestampValidBits;
+  VkExtent3D minImageTransferGranularity;
+} VkQueueFamilyProperties;
+
+typedef struct VkMemoryType {
+  VkMemoryPropertyFlags propertyFlags;
+  uint32_t heapIndex;
+} VkMemoryType;
+
+typedef struct VkMemoryHeap {
+  VkDeviceSize size;
+  VkMemoryHeapFlags flags;
+} VkMemoryHeap;
+
+typedef struct VkPhysicalDeviceMemoryProperties {
+  uint32_t memoryTypeCount;
+  VkMemoryType memoryTypes[VK_MAX_MEMORY_TYPES];
+  uint32_t memoryHeapCount;
+  VkMemoryHeap memoryHeaps[VK_MAX_MEMORY_HEAPS];
+} VkPhysicalDeviceMemoryProperties;
+
+typedef struct VkFormatProperties {
+  VkFormatFeatureFlags linearTilingFeatures;
+  VkFormatFeatureFlags optimalTilingFeatures;
+  VkFormatFeatureFlags bufferFeatures;
+} VkFormatProperties;
+
+typedef struct VkExtensionProperties {
+  char extensionName[VK_MAX_EXTENSION_NAME_SIZE];
+  uint32_t specVersion;
+} VkExtensionProperties;
+
+typedef struct VkDeviceQueueCreateInfo {
+  VkStructureType sType;
+  const void *pNext;
+  VkDeviceQueueCreateFlags flags;
+  uint32_t queueFamilyIndex;
+  uint32_t queueCount;
+  const float *pQueuePriorities;
+} VkDeviceQueueCreateInfo;
+
+typedef struct VkDeviceCreateInfo {
+  VkStructureType sType;
+  const void *pNext;
+  VkDeviceCreateFlags flags;
+  uint32_t queueCreateInfoCount;
+  const VkDeviceQueueCreateInfo *pQueueCreateInfos;
+  uint32_t enabledLayerCount;
+  const char *const *ppEnabledLayerNames;
+  uint32_t enabledExtensionCount;
+  const char *const *ppEnabledExtensionNames;
+  const VkPhysicalDeviceFeatures *pEnabledFeatures;
+} VkDeviceCreateInfo;
+
+typedef struct VkSubmitInfo {
+  VkStructureType sType;
+  const void *pNext;
+  uint32_t waitSemaphoreCount;
+  const VkSemaphore *pWaitSemaphores;
+  const VkPipelineStageFlags *pWaitDstStageMask;
+  uint32_t commandBufferCount;
+  const VkCommandBuffer *pCommandBuffers;
+  uint32_t signalSemaphoreCount;
+  const VkSemaphore *pSignalSemaphores;
+} VkSubmitInfo;
+
+typedef struct VkMemoryAllocateInfo {
+  VkStructureType sType;
+  const void *pNext;
+  VkDeviceSize allocationSize;
+  uint32_t memoryTypeIndex;
+} VkMemoryAllocateInfo;
+
+typedef struct VkMappedMemoryRange {
+  VkStructureType sType;
+  const void *pNext;
+  VkDeviceMemory memory;
+  VkDeviceSize offset;
+  VkDeviceSize size;
+} VkMappedMemoryRange;
+
+typedef struct VkMemoryRequirements {
+  VkDeviceSize size;
+  VkDeviceSize alignment;
+  uint32_t memoryTypeBits;
+} VkMemoryRequirements;
+
+typedef struct VkFenceCreateInfo {
+  VkStructureType sType;
+  const void *pNext;
+  VkFenceCreateFlags flags;
+} VkFenceCreateInfo;
+
+typedef struct VkSemaphoreCreateInfo {
+  VkStructureType sType;
+  const void *pNext;
+  VkSemaphoreCreateFlags flags;
+} VkSemaphoreCreateInfo;
+
+typedef struct VkBufferCreateInfo {
+  VkStructureType sType;
+  const void *pNext;
+  VkBufferCreateFlags flags;
+  VkDeviceSize size;
+  VkBufferUsageFlags usage;
+  VkSharingMode sharingMode;
+  uint32_t queueFamilyIndexCount;
+  const uint32_t *pQueueFamilyIndices;
+} VkBufferCreateInfo;
+
+typedef struct VkImageCreateInfo {
+  VkStructureType sType;
+  const void *pNext;
+  VkImageCreateFlags flags;
+  VkImageType imageType;
+  VkFormat format;
+  VkExtent3D extent;
+  uint32_t mipLevels;
+  uint32_t arrayLayers;
+  VkSampleCountFlagBits samples;
+  VkImageTiling tiling;
+  VkImageUsageFlags usage;
+  VkSharingMode sharingMode;
+  uint32_t queueFamilyIndexCount;
+  const uint32_t *pQueueFamilyIndices;
+  VkImageLayout initialLayout;
+} VkImageCreateInfo;
+
+typedef struct VkImageSubresource {
+  VkImageAspectFlags aspectMask;
+  uint32_t mipLevel;
+  uint32_t arrayLayer;
+} VkImageSubresource;
+
+typedef struct VkSubresourceLayout {
+  VkDeviceSize offset;
+  VkDeviceSize size;
+  VkDeviceSize rowPitch;
+  VkDeviceSize arrayPitch;
+  VkDeviceSize depthPitch;
+} VkSubresourceLayout;
+
+typedef struct VkComponentMapping {
+  VkComponentSwizzle r, g, b, a;
+} VkComponentMapping;
+
+typedef struct VkImageSubresourceRange {
+  VkImageAspectFlags aspectMask;
+  uint32_t baseMipLevel;
+  uint32_t levelCount;
+  uint32_t baseArrayLayer;
+  uint32_t layerCount;
+} VkImageSubresourceRange;
+
+typedef struct VkImageViewCreateInfo {
+  VkStructureType sType;
+  const void *pNext;
+  VkImageViewCreateFlags flags;
+  VkImage image;
+  VkImageViewType viewType;
+  VkFormat format;
+  VkComponentMapping components;
+  VkImageSubresourceRange subresourceRange;
+} VkImageViewCreateInfo;
+
+typedef struct VkShaderModuleCreateInfo {
+  VkStructureType sType;
+  const void *pNext;
+  VkShaderModuleCreateFlags flags;
+  size_t codeSize;
+  const uint32_t *pCode;
+} VkShaderModuleCreateInfo;
+
+typedef struct VkPipelineCacheCreateInfo {
+  VkStructureType sType;
+  const void *pNext;
+  VkPipelineCacheCreateFlags flags;
+  size_t initialDataSize;
+  const void *pInitialData;
+} VkPipelineCacheCreateInfo;
+
+typedef struct VkSpecializationMapEntry {
+  uint32_t constantID;
+  uint32_t offset;
+  size_t size;
+} VkSpecializationMapEntry;
+
+typedef struct VkSpecializationInfo {
+  uint32_t mapEntryCount;
+  const VkSpecializationMapEntry *pMapEntries;
+  size_t dataSize;
+  const void *pData;
+} VkSpecializationInfo;
+
+typedef struct VkPipelineShaderStageCreateInfo {
+  VkStructureType sType;
+  const void *pNext;
+  VkPipelineShaderStageCreateFlags flags;
+  VkShaderStageFlagBits stage;
+  VkShaderModule module;
+  const char *pName;
+  const VkSpecializationInfo *pSpecializationInfo;
+} VkPipelineShaderStageCreateInfo;
+
+typedef struct VkVertexInputBindingDescription {
+  uint32_t binding;
+  uint32_t stride;
+  VkVertexInputRate inputRate;
+} VkVertexInputBindingDescription;
+
+typedef struct VkVertexInputAttributeDescription {
+  uint32_t location;
+  uint32_t binding;
+  VkFormat format;
+  uint32_t offset;
+} VkVertexInputAttributeDescription;
+
+typedef struct VkPipelineVertexInputStateCreateInfo {
+  VkStructureType sType;
+  const void *pNext;
+  VkPipelineVertexInputStateCreateFlags flags;
+  uint32_t vertexBindingDescriptionCount;
+  const VkVertexInputBindingDescription *pVertexBindingDescriptions;
+  uint32_t vertexAttributeDescriptionCount;
+  const VkVertexInputAttributeDescription *pVertexAttributeDescriptions;
+} VkPipelineVertexInputStateCreateInfo;
+
+typedef struct VkPipelineInputAssemblyStateCreateInfo {
+  VkStructureType sType;
+  const void *pNext;
+  VkPipelineInputAssemblyStateCreateFlags flags;
+  VkPrimitiveTopology topology;
+  VkBool32 primitiveRestartEnable;
+} VkPipelineInputAssemblyStateCreateInfo;
+
+typedef struct VkPipelineTessellationStateCreateInfo {
+  VkStructureType sType;
+  const void *pNext;
+  VkPipelineTessellationStateCreateFlags flags;
+  uint32_t patchControlPoints;
+} VkPipelineTessellationStateCreateInfo;
+
+typedef struct VkViewport {
+  float x, y, width, height, minDepth, maxDepth;
+} VkViewport;
+
+typedef struct VkPipelineViewportStateCreateInfo {
+  VkStructureType sType;
+  const void *pNext;
+  VkPipelineViewportStateCreateFlags flags;
+  uint32_t viewportCount;
+  const VkViewport *pViewports;
+  uint32_t scissorCount;
+  const VkRect2D *pScissors;
+} VkPipelineViewportStateCreateInfo;
+
+typedef struct VkPipelineRasterizationStateCreateInfo {
+  VkStructureType sType;
+  const void *pNext;
+  VkPipelineRasterizationStateCreateFlags flags;
+  VkBool32 depthClampEnable;
+  VkBool32 rasterizerDiscardEnable;
+  VkPolygonMode polygonMode;
+  VkCullModeFlags cullMode;
+  VkFrontFace frontFace;
+  VkBool32 depthBiasEnable;
+  float depthBiasConstantFactor;
+  float depthBiasClamp;
+  float depthBiasSlopeFactor;
+  float lineWidth;
+} VkPipelineRasterizationStateCreateInfo;
+
+typedef struct VkPipelineMultisampleStateCreateInfo {
+  VkStructureType sType;
+  const void *pNext;
+  VkPipelineMultisampleStateCreateFlags flags;
+  VkSampleCountFlagBits rasterizationSamples;
+  VkBool32 sampleShadingEnable;
+  float minSampleShading;
+  const VkSampleMask *pSampleMask;
+  VkBool32 alphaToCoverageEnable;
+  VkBool32 alphaToOneEnable;
+} VkPipelineMultisampleStateCreateInfo;
+
+typedef struct VkStencilOpState {
+  VkStencilOp failOp;
+  VkStencilOp passOp;
+  VkStencilOp depthFailOp;
+  VkCompareOp compareOp;
+  uint32_t compareMask;
+  uint32_t writeMask;
+  uint32_t reference;
+} VkStencilOpState;
+
+typedef struct VkPipelineDepthStencilStateCreateInfo {
+  VkStructureType sType;
+  const void *pNext;
+  VkPipelineDepthStencilStateCreateFlags flags;
+  VkBool32 depthTestEnable;
+  VkBool32 depthWriteEnable;
+  VkCompareOp depthCompareOp;
+  VkBool32 depthBoundsTestEnable;
+  VkBool32 stencilTestEnable;
+  VkStencilOpState front;
+  VkStencilOpState back;
+  float minDepthBounds;
+  float maxDepthBounds;
+} VkPipelineDepthStencilStateCreateInfo;
+
+typedef struct VkPipelineColorBlendAttachmentState {
+  VkBool32 blendEnable;
+  VkBlendFactor srcColorBlendFactor;
+  VkBlendFactor dstColorBlendFactor;
+  VkBlendOp colorBlendOp;
+  VkBlendFactor srcAlphaBlendFactor;
+  VkBlendFactor dstAlphaBlendFactor;
+  VkBlendOp alphaBlendOp;
+  VkColorComponentFlags colorWriteMask;
+} VkPipelineColorBlendAttachmentState;
+
+typedef struct VkPipelineColorBlendStateCreateInfo {
+  VkStructureType sType;
+  const void *pNext;
+  VkPipelineColorBlendStateCreateFlags flags;
+  VkBool32 logicOpEnable;
+  VkLogicOp logicOp;
+  uint32_t attachmentCount;
+  const VkPipelineColorBlendAttachmentState *pAttachments;
+  float blendConstants[4];
+} VkPipelineColorBlendStateCreateInfo;
+
+typedef struct VkPipelineDynamicStateCreateInfo {
+  VkStructureType sType;
+  const void *pNext;
+  VkPipelineDynamicStateCreateFlags flags;
+  uint32_t dynamicStateCount;
+  const VkDynamicState *pDynamicStates;
+} VkPipelineDynamicStateCreateInfo;
+
+typedef struct VkGraphicsPipelineCreateInfo {
+  VkStructureType sType;
+  const void *pNext;
+  VkPipelineCreateFlags flags;
+  uint32_t stageCount;
+  const VkPipelineShaderStageCreateInfo *pStages;
+  const VkPipelineVertexInputStateCreateInfo *pVertexInputState;
+  const VkPipelineInputAssemblyStateCreateInfo *pInputAssemblyState;
+  const VkPipelineTessellationStateCreateInfo *pTessellationState;
+  const VkPipelineViewportStateCreateInfo *pViewportState;
+  const VkPipelineRasterizationStateCreateInfo *pRasterizationState;
+  const VkPipelineMultisampleStateCreateInfo *pMultisampleState;
+  const VkPipelineDepthStencilStateCreateInfo *pDepthStencilState;
+  const VkPipelineColorBlendStateCreateInfo *pColorBlendState;
+  const VkPipelineDynamicStateCreateInfo *pDynamicState;
+  VkPipelineLayout layout;
+  VkRenderPass renderPass;
+  uint32_t subpass;
+  VkPipeline basePipelineHandle;
+  int32_t basePipelineIndex;
+} VkGraphicsPipelineCreateInfo;
+
+typedef struct VkComputePipelineCreateInfo {
+  VkStructureType sType;
+  const void *pNext;
+  VkPipelineCreateFlags flags;
+  VkPipelineShaderStageCreateInfo stage;
+  VkPipelineLayout layout;
+  VkPipeline basePipelineHandle;
+  int32_t basePipelineIndex;
+} VkComputePipelineCreateInfo;
+
+typedef struct VkPushConstantRange {
+  VkShaderStageFlags stageFlags;
+  uint32_t offset;
+  uint32_t size;
+} VkPushConstantRange;
+
+typedef struct VkPipelineLayoutCreateInfo {
+  VkStructureType sType;
+  const void *pNext;
+  VkPipelineLayoutCreateFlags flags;
+  uint32_t setLayoutCount;
+  const VkDescriptorSetLayout *pSetLayouts;
+  uint32_t pushConstantRangeCount;
+  const VkPushConstantRange *pPushConstantRanges;
+} VkPipelineLayoutCreateInfo;
+
+typedef struct VkSamplerCreateInfo {
+  VkStructureType sType;
+  const void *pNext;
+  VkSamplerCreateFlags flags;
+  VkFilter magFilter;
+  VkFilter minFilter;
+  VkSamplerMipmapMode mipmapMode;
+  VkSamplerAddressMode addressModeU;
+  VkSamplerAddressMode addressModeV;
+  VkSamplerAddressMode addressModeW;
+  float mipLodBias;
+  VkBool32 anisotropyEnable;
+  float maxAnisotropy;
+  VkBool32 compareEnable;
+  VkCompareOp compareOp;
+  float minLod;
+  float maxLod;
+  VkBorderColor borderColor;
+  VkBool32 unnormalizedCoordinates;
+} VkSamplerCreateInfo;
+
+typedef struct VkDescriptorSetLayoutBinding {
+  uint32_t binding;
+  VkDescriptorType descriptorType;
+  uint32_t descriptorCount;
+  VkShaderStageFlags stageFlags;
+  const VkSampler *pImmutableSamplers;
+} VkDescriptorSetLayoutBinding;
+
+typedef struct VkDescriptorSetLayoutCreateInfo {
+  VkStructureType sType;
+  const void *pNext;
+  VkDescriptorSetLayoutCreateFlags flags;
+  uint32_t bindingCount;
+  const VkDescriptorSetLayoutBinding *pBindings;
+} VkDescriptorSetLayoutCreateInfo;
+
+typedef struct VkDescriptorPoolSize {
+  VkDescriptorType type;
+  uint32_t descriptorCount;
+} VkDescriptorPoolSize;
+
+typedef struct VkDescriptorPoolCreateInfo {
+  VkStructureType sType;
+  const void *pNext;
+  VkDescriptorPoolCreateFlags flags;
+  uint32_t maxSets;
+  uint32_t poolSizeCount;
+  const VkDescriptorPoolSize *pPoolSizes;
+} VkDescriptorPoolCreateInfo;
+
+typedef struct VkDescriptorSetAllocateInfo {
+  VkStructureType sType;
+  const void *pNext;
+  VkDescriptorPool descriptorPool;
+  uint32_t descriptorSetCount;
+  const VkDescriptorSetLayout *pSetLayouts;
+} VkDescriptorSetAllocateInfo;
+
+typedef struct VkDescriptorImageInfo {
+  VkSampler sampler;
+  VkImageView imageView;
+  VkImageLayout imageLayout;
+} VkDescriptorImageInfo;
+
+typedef struct VkDescriptorBufferInfo {
+  VkBuffer buffer;
+  VkDeviceSize offset;
+  VkDeviceSize range;
+} VkDescriptorBufferInfo;
+
+typedef struct VkWriteDescriptorSet {
+  VkStructureType sType;
+  const void *pNext;
+  VkDescriptorSet dstSet;
+  uint32_t dstBinding;
+  uint32_t dstArrayElement;
+  uint32_t descriptorCount;
+  VkDescriptorType descriptorType;
+  const VkDescriptorImageInfo *pImageInfo;
+  const VkDescriptorBufferInfo *pBufferInfo;
+  const VkBufferView *pTexelBufferView;
+} VkWriteDescriptorSet;
+
+typedef struct VkCopyDescriptorSet {
+  VkStructureType sType;
+  const void *pNext;
+  VkDescriptorSet srcSet;
+  uint32_t srcBinding;
+  uint32_t srcArrayElement;
+  VkDescriptorSet dstSet;
+  uint32_t dstBinding;
+  uint32_t dstArrayElement;
+  uint32_t descriptorCount;
+} VkCopyDescriptorSet;
+
+typedef struct VkFramebufferCreateInfo {
+  VkStructureType sType;
+  const void *pNext;
+  VkFramebufferCreateFlags flags;
+  VkRenderPass renderPass;
+  uint32_t attachmentCount;
+  const VkImageView *pAttachments;
+  uint32_t width;
+  uint32_t height;
+  uint32_t layers;
+} VkFramebufferCreateInfo;
+
+typedef struct VkAttachmentDescription {
+  VkAttachmentDescriptionFlags flags;
+  VkFormat format;
+  VkSampleCountFlagBits samples;
+  VkAttachmentLoadOp loadOp;
+  VkAttachmentStoreOp storeOp;
+  VkAttachmentLoadOp stencilLoadOp;
+  VkAttachmentStoreOp stencilStoreOp;
+  VkImageLayout initialLayout;
+  VkImageLayout finalLayout;
+} VkAttachmentDescription;
+
+typedef struct VkAttachmentReference {
+  uint32_t attachment;
+  VkImageLayout layout;
+} VkAttachmentReference;
+
+typedef struct VkSubpassDescription {
+  VkSubpassDescriptionFlags flags;
+  VkPipelineBindPoint pipelineBindPoint;
+  uint32_t inputAttachmentCount;
+  const VkAttachmentReference *pInputAttachments;
+  uint32_t colorAttachmentCount;
+  const VkAttachmentReference *pColorAttachments;
+  const VkAttachmentReference *pResolveAttachments;
+  const VkAttachmentReference *pDepthStencilAttachment;
+  uint32_t preserveAttachmentCount;
+  const uint32_t *pPreserveAttachments;
+} VkSubpassDescription;
+
+typedef struct VkSubpassDependency {
+  uint32_t srcSubpass;
+  uint32_t dstSubpass;
+  VkPipelineStageFlags srcStageMask;
+  VkPipelineStageFlags dstStageMask;
+  VkAccessFlags srcAccessMask;
+  VkAccessFlags dstAccessMask;
+  VkDependencyFlags dependencyFlags;
+} VkSubpassDependency;
+
+typedef struct VkRenderPassCreateInfo {
+  VkStructureType sType;
+  const void *pNext;
+  VkRenderPassCreateFlags flags;
+  uint32_t attachmentCount;
+  const VkAttachmentDescription *pAttachments;
+  uint32_t subpassCount;
+  const VkSubpassDescription *pSubpasses;
+  uint32_t dependencyCount;
+  const VkSubpassDependency *pDependencies;
+} VkRenderPassCreateInfo;
+
+typedef struct VkCommandPoolCreateInfo {
+  VkStructureType sType;
+  const void *pNext;
+  VkCommandPoolCreateFlags flags;
+  uint32_t queueFamilyIndex;
+} VkCommandPoolCreateInfo;
+
+typedef struct VkCommandBufferAllocateInfo {
+  VkStructureType sType;
+  const void *pNext;
+  VkCommandPool commandPool;
+  VkCommandBufferLevel level;
+  uint32_t commandBufferCount;
+} VkCommandBufferAllocateInfo;
+
+typedef struct VkCommandBufferInheritanceInfo {
+  VkStructureType sType;
+  const void *pNext;
+  VkRenderPass renderPass;
+  uint32_t subpass;
+  VkFramebuffer framebuffer;
+  VkBool32 occlusionQueryEnable;
+  VkQueryControlFlags queryFlags;
+  VkQueryPipelineStatisticFlags pipelineStatistics;
+} VkCommandBufferInheritanceInfo;
+
+typedef struct VkCommandBufferBeginInfo {
+  VkStructureType sType;
+  const void *pNext;
+  VkCommandBufferUsageFlags flags;
+  const VkCommandBufferInheritanceInfo *pInheritanceInfo;
+} VkCommandBufferBeginInfo;
+
+typedef union VkClearColorValue {
+  float float32[4];
+  int32_t int32[4];
+  uint32_t uint32[4];
+} VkClearColorValue;
+
+typedef struct VkClearDepthStencilValue {
+  float depth;
+  uint32_t stencil;
+} VkClearDepthStencilValue;
+
+typedef union VkClearValue {
+  VkClearColorValue color;
+  VkClearDepthStencilValue depthStencil;
+} VkClearValue;
+
+typedef struct VkRenderPassBeginInfo {
+  VkStructureType sType;
+  const void *pNext;
+  VkRenderPass renderPass;
+  VkFramebuffer framebuffer;
+  VkRect2D renderArea;
+  uint32_t clearValueCount;
+  const VkClearValue *pClearValues;
+} VkRenderPassBeginInfo;
+
+typedef struct VkImageSubresourceLayers {
+  VkImageAspectFlags aspectMask;
+  uint32_t mipLevel;
+  uint32_t baseArrayLayer;
+  uint32_t layerCount;
+} VkImageSubresourceLayers;
+
+typedef struct VkBufferImageCopy {
+  VkDeviceSize bufferOffset;
+  uint32_t bufferRowLength;
+  uint32_t bufferImageHeight;
+  VkImageSubresourceLayers imageSubresource;
+  VkOffset3D imageOffset;
+  VkExtent3D imageExtent;
+} VkBufferImageCopy;
+
+typedef struct VkBufferCopy {
+  VkDeviceSize srcOffset;
+  VkDeviceSize dstOffset;
+  VkDeviceSize size;
+} VkBufferCopy;
+
+typedef struct VkMemoryBarrier VkMemoryBarrier;             /* recorded as a no-op */
+typedef struct VkBufferMemoryBarrier VkBufferMemoryBarrier; /* recorded as a no-op */
+typedef struct VkImageMemoryBarrier VkImageMemoryBarrier;   /* recorded as a no-op */
+
+typedef struct VkSurfaceCapabilitiesKHR {
+  uint32_t minImageCount;
+  uint32_t maxImageCount;
+  VkExtent2D currentExtent;
+  VkExtent2D minImageExtent;
+  VkExtent2D maxImageExtent;
+  uint32_t maxImageArrayLayers;
+  VkSurfaceTransformFlagsKHR supportedTransforms;
+  VkSurfaceTransformFlagBitsKHR currentTransform;
+  VkCompositeAlphaFlagsKHR supportedCompositeAlpha;
+  VkImageUsageFlags supportedUsageFlags;
+} VkSurfaceCapabilitiesKHR;
+
+typedef struct VkSurfaceFormatKHR {
+  VkFormat format;
+  VkColorSpaceKHR colorSpace;
+} VkSurfaceFormatKHR;
+
+typedef struct VkSwapchainCreateInfoKHR {
+  VkStructureType sType;
+  const void *pNext;
+  VkSwapchainCreateFlagsKHR flags;
+  VkSurfaceKHR surface;
+  uint32_t minImageCount;
+  VkFormat imageFormat;
+  VkColorSpaceKHR imageColorSpace;
+  VkExtent2D imageExtent;
+  uint32_t imageArrayLayers;
+  VkImageUsageFlags imageUsage;
+  VkSharingMode imageSharingMode;
+  uint32_t queueFamilyIndexCount;
+  const uint32_t *pQueueFamilyIndices;
+  VkSurfaceTransformFlagBitsKHR preTransform;
+  VkCompositeAlphaFlagBitsKHR compositeAlpha;
+  VkPresentModeKHR presentMode;
+  VkBool32 clipped;
+  VkSwapchainKHR oldSwapchain;
+} VkSwapchainCreateInfoKHR;
+
+typedef struct VkPresentInfoKHR {
+  VkStructureType sType;
+  const void *pNext;
+  uint32_t waitSemaphoreCount;
+  const VkSemaphore *pWaitSemaphores;
+  uint32_t swapchainCount;
+  const VkSwapchainKHR *pSwapchains;
+  const uint32_t *pImageIndices;
+  VkResult *pResults;
+} VkPresentInfoKHR;
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* VISOR_VK_H */

@@ -1,0 +1,144 @@
+/*
+ * refrast.h - ORACLE (test infrastructure only).
+ *
+ * A plain-C restatement of the reference's draw-execution path (reference rasterizer.cpp,
+ * texture_sampling.cpp, precompiled.cpp and the shader lowering rules of spirv_compile.cpp).
+ * It is the checker the GPU path is compared against and the timed CPU baseline. Only tests,
+ * the smoke check and the benchmark's CPU-baseline leg may load it; the product never does.
+ *
+ * The reference itself cannot be built here (it needs the MSVC dialect and CRT, and LLVM 6 for
+ * its shader JIT), so every function below cites the reference lines it restates, and the tests
+ * pin it with closed forms, brute force and special cases.
+ */
+#ifndef VISOR_REFRAST_H
+#define VISOR_REFRAST_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Vulkan enum values used below (Vulkan 1.0 spec). */
+enum {
+  RR_TOPO_TRIANGLE_LIST = 3,
+  RR_TOPO_TRIANGLE_STRIP = 4,
+  RR_FRONT_CCW = 0,
+  RR_FRONT_CW = 1,
+  RR_CULL_FRONT = 1,
+  RR_CULL_BACK = 2,
+  RR_CMP_NEVER = 0,
+  RR_CMP_LESS = 1,
+  RR_CMP_EQUAL = 2,
+  RR_CMP_LEQUAL = 3,
+  RR_CMP_GREATER = 4,
+  RR_CMP_NOTEQUAL = 5,
+  RR_CMP_GEQUAL = 6,
+  RR_CMP_ALWAYS = 7,
+  RR_BF_ZERO = 0,
+  RR_BF_ONE = 1,
+  RR_BF_SRC_ALPHA = 6,
+  RR_BF_ONE_MINUS_SRC_ALPHA = 7,
+  RR_BLEND_OP_ADD = 0,
+  RR_FMT_R8G8B8A8_UNORM = 37,
+  RR_FMT_B8G8R8A8_UNORM = 44,
+  RR_FMT_R16G16B16A16_SFLOAT = 97,
+  RR_FMT_R32_SFLOAT = 100,
+  RR_FMT_R32G32_SFLOAT = 103,
+  RR_FMT_R32G32B32_SFLOAT = 106,
+  RR_FMT_R32G32B32A32_SFLOAT = 109,
+  RR_FMT_D32_SFLOAT = 126,
+  RR_FILTER_NEAREST = 0,
+  RR_FILTER_LINEAR = 1,
+  RR_MIPMAP_NEAREST = 0,
+  RR_MIPMAP_LINEAR = 1,
+  RR_ADDR_REPEAT = 0,
+  RR_ADDR_CLAMP_TO_EDGE = 2,
+};
+
+/* Built-in restated shader programs. Each mirrors one SPIR-V module the package's shader
+ * library emits (visor_amd/shaders.py), lowered by the reference's rules. */
+enum {
+  RR_VS_MVP_POS4_COL = 1,     /* gl_Position = pc.mvp * pos; out0 = col               */
+  RR_VS_MVP_NORMAL_UV = 2,    /* gl_Position = pc.mvp * vec4(pos,1); out0 = pc.nmat*nrm; out1 = uv */
+  RR_FS_COLOR = 101,          /* out = in0                                            */
+  RR_FS_LAMBERT_TEX = 102,    /* normalize, max(dot), texture(uv) * light, alpha 1    */
+  RR_FS_LAMBERT_TEX_A = 103,  /* as above with alpha = pc.alpha (push constant 112)  */
+  RR_FS_MULTITAP8 = 104,      /* 8 array-texture taps accumulated in FP32             */
+  RR_FS_TEX_ONLY = 105,       /* out = texture(tex, in1.xy)                           */
+};
+
+typedef struct rr_image {
+  uint8_t *pixels;          /* linear rows; mips packed within a layer, then layers     */
+  uint32_t width, height;
+  uint32_t mip_levels, array_layers;
+  uint32_t bpp;             /* bytes per pixel: 4 (RGBA8/BGRA8/D32) or 8 (RGBA16F)      */
+  uint32_t format;
+} rr_image;
+
+typedef struct rr_sampler {
+  uint32_t mag_filter, min_filter, mipmap_mode, address_u, address_v;
+  float mip_lod_bias, min_lod, max_lod;
+} rr_sampler;
+
+typedef struct rr_vattr {
+  uint32_t format, stride, offset, binding;
+} rr_vattr;
+
+typedef struct rr_pipeline {
+  rr_vattr vattrs[16];
+  uint32_t topology, front_face, cull_mode;
+  uint32_t depth_compare_op, depth_write;
+  uint32_t blend_enable, src_color_factor, dst_color_factor, color_blend_op;
+  uint32_t vs_program, fs_program;
+} rr_pipeline;
+
+typedef struct rr_state {
+  const uint8_t *ib;        /* index buffer bytes + bound offset                        */
+  uint32_t index_type;      /* 0 = u16, 1 = u32                                         */
+  const uint8_t *vbs[4];    /* vertex buffer bytes + bound offset, per binding slot     */
+  rr_image *col0;
+  rr_image *depth;          /* NULL when the render pass has no depth attachment        */
+  const rr_pipeline *pipeline;
+  const rr_image *tex[4];   /* set 0, binding i                                         */
+  const rr_sampler *samp[4];
+  uint8_t pushconsts[128];
+} rr_state;
+
+typedef struct rr_counters {
+  uint64_t draws, tris_in, tris_out, blocks, pixels_tested, pixels_written, depth_passed;
+} rr_counters;
+
+/* Vertex shader output (reference gpu.h:53-57). */
+typedef struct rr_vertex {
+  float position[4];
+  float interps[10][4];
+} rr_vertex;
+
+void rr_clear_color(rr_image *target, const float rgba[4]);
+void rr_clear_depth(rr_image *target, float depth);
+
+/* Serial, API-ordered draw: the parity mode. */
+void rr_draw(const rr_state *state, int num_verts, uint32_t first, int indexed, rr_counters *ctr);
+
+/* Reference-structured threaded draw (nthreads workers + a work-stealing submitter over a
+ * mutex-guarded FIFO of 32x32 blocks). Racy by design like the reference; used for timing. */
+void rr_draw_threaded(const rr_state *state, int num_verts, uint32_t first, int indexed,
+                      int nthreads, rr_counters *ctr);
+
+/* Exposed pieces, for the pinning tests. */
+void rr_to_window(uint32_t w, uint32_t h, const float pos[4], int32_t out[2]);
+void rr_sample(const rr_image *img, const rr_sampler *samp, const float coord[3],
+               const float ddx[2], const float ddy[2], int use_lod, float out[4]);
+void rr_sample_tex_wrapped(float u, float v, const rr_image *tex, uint64_t byte_offs, float out[4]);
+float rr_log2(float x);
+uint16_t rr_f32_to_f16(float f);
+float rr_f16_to_f32(uint16_t h);
+uint64_t rr_subresource_offset(const rr_image *img, uint32_t mip, uint32_t layer);
+void rr_get_vertex_attribute(const rr_state *state, uint32_t vtx, uint32_t attr, float out[4]);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,871 @@
+// exec.cpp - command buffer execution on the GPU.
+//
+// Decodes the recorded stream (reference cmd_exec.cpp:15-185) into render-pass and copy
+// operations with a per-draw state snapshot, then runs the draw-execution path:
+//   plain    per draw: setup, bin, raster, then wait for the GPU (as the reference's blocking
+//            DrawTriangles, rasterizer.cpp:363-520); BeginRenderPass clears with fill kernels.
+//   batched  every draw of a render pass in one setup / bin / raster sequence with the clear
+//            fused into the raster kernel's first touch of each tile.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <set>
+
+#include "internal.h"
+#include "runtime_kernels.h"
+
+namespace visor
+{
+#define HIPCHK(x)                                                                     \
+  do                                                                                  \
+  {                                                                                   \
+    hipError_t e_ = (x);                                                              \
+    if(e_ != hipSuccess)                                                              \
+    {                                                                                 \
+      LogError("%s failed: %s (%s:%d)", #x, hipGetErrorString(e_), __FILE__, __LINE__); \
+      return HipToVk(e_);                                                             \
+    }                                                                                 \
+  } while(0)
+
+static uint32_t TileShiftFromEnv()
+{
+  const char *t = getenv("VISOR_TILE");
+  if(t)
+  {
+    int v = atoi(t);
+    if(v == 8) return 3;
+    if(v == 32) return 5;
+  }
+  return 4;
+}
+
+Queue::Queue()
+{
+  const char *m = getenv("VISOR_EXEC_MODE");
+  if(m && !strcmp(m, "plain"))
+    mode = ExecMode::Plain;
+  tiles.tile_shift = TileShiftFromEnv();
+  const char *a = getenv("VISOR_SUBMIT");
+  async = a && !strcmp(a, "async");
+}
+
+Queue::~Queue()
+{
+  if(worker.joinable())
+  {
+    {
+      std::lock_guard<std::mutex> lk(qm);
+      stop = true;
+    }
+    qcv.notify_all();
+    worker.join();
+  }
+  void *bufs[] = {d_frames, d_draws, d_rec, d_count, d_offs, d_keys, d_vals, d_keys2, d_vals2, d_start, d_end, d_ctr, d_temp};
+  for(void *b : bufs)
+    if(b)
+      hipFree(b);
+  if(h_pinned)
+    hipHostFree(h_pinned);
+  for(auto &e : ev)
+    if(e)
+      hipEventDestroy(e);
+  if(stream)
+    hipStreamDestroy(stream);
+}
+
+bool Queue::Grow(void **p, size_t &cap, size_t need)
+{
+  if(need <= cap && *p)
+    return true;
+  size_t ncap = std::max(need + need / 2, (size_t)4096);
+  if(*p)
+  {
+    hipStreamSynchronize(stream);
+    hipFree(*p);
+    *p = nullptr;
+  }
+  if(hipMalloc(p, ncap) != hipSuccess)
+  {
+    cap = 0;
+    *p = nullptr;
+    return false;
+  }
+  cap = ncap;
+  return true;
+}
+
+static uint64_t ImageBytes(const VkImage_T *img)
+{
+  uint64_t slice = 0;
+  uint32_t w = img->extent.width, h = img->extent.height;
+  for(uint32_t m = 0; m < img->mipLevels; m++)
+  {
+    slice += (uint64_t)w * h * img->bytesPerPixel;
+    w = std::max(1u, w >> 1);
+    h = std::max(1u, h >> 1);
+  }
+  return slice * img->arrayLayers;
+}
+
+// reference precompiled.cpp:3-36 (CalcSubresourceByteOffset)
+static uint64_t SubresourceOffset(const VkImage_T *img, uint32_t mip, uint32_t layer)
+{
+  uint64_t offs = 0;
+  const uint32_t w = img->extent.width, h = img->extent.height, bpp = img->bytesPerPixel;
+  for(uint32_t m = 0; m < mip; m++)
+    offs += (uint64_t)std::max(1u, w >> m) * std::max(1u, h >> m) * bpp;
+  if(layer > 0)
+  {
+    uint64_t slice = 0;
+    uint32_t mw = w, mh = h;
+    for(uint32_t m = 0; m < img->mipLevels; m++)
+    {
+      slice += (uint64_t)mw * mh * bpp;
+      mw = std::max(1u, mw >> 1);
+      mh = std::max(1u, mh >> 1);
+    }
+    offs += slice * layer;
+  }
+  return offs;
+}
+
+static void FillTex(VaTex &t, const VkImage_T *img, const VkSampler_T *s)
+{
+  memset(&t, 0, sizeof(t));
+  t.base = (unsigned long long)(uintptr_t)img->devPixels();
+  t.width = img->extent.width;
+  t.height = img->extent.height;
+  t.mips = std::max(1u, std::min<uint32_t>(img->mipLevels, VA_MAX_LEVELS));
+  t.layers = std::max(1u, img->arrayLayers);
+  t.bpp = img->bytesPerPixel;
+  uint64_t off = 0;
+  uint32_t w = t.width, h = t.height;
+  for(uint32_t m = 0; m < img->mipLevels; m++)
+  {
+    if(m < VA_MAX_LEVELS)
+      t.level_off[m] = (unsigned)off;
+    off += (uint64_t)w * h * img->bytesPerPixel;
+    w = std::max(1u, w >> 1);
+    h = std::max(1u, h >> 1);
+  }
+  t.slice_bytes = off;
+  if(s)
+  {
+    const VkSamplerCreateInfo &i = s->info;
+    t.mag_filter = i.magFilter;
+    t.min_filter = i.minFilter;
+    t.mipmap_mode = i.mipmapMode;
+    t.addr_u = i.addressModeU == VK_SAMPLER_ADDRESS_MODE_REPEAT ? 0u : 2u;
+    t.addr_v = i.addressModeV == VK_SAMPLER_ADDRESS_MODE_REPEAT ? 0u : 2u;
+    t.lod_bias = i.mipLodBias;
+    t.min_lod = i.minLod;
+    t.max_lod = i.maxLod;
+  }
+  else
+  {
+    // the reference's only sampling mode: bilinear, wrap, level 0 (texture_sampling.cpp:139-184)
+    t.mag_filter = t.min_filter = 1;
+    t.mipmap_mode = 0;
+    t.addr_u = t.addr_v = 0;
+    t.min_lod = t.max_lod = 0.0f;
+  }
+}
+
+// bytes of a buffer readable from offset, limited by both the buffer and its memory
+static unsigned long long BoundBytes(const VkBuffer_T *b, VkDeviceSize off)
+{
+  if(!b || !b->mem)
+    return 0;
+  uint64_t end = std::min<uint64_t>(b->size, b->mem->size > b->offset ? b->mem->size - b->offset : 0);
+  return off < end ? end - off : 0;
+}
+
+static void AddMem(std::vector<VkDeviceMemory_T *> &v, VkDeviceMemory_T *m)
+{
+  if(m && m->type == 1 && std::find(v.begin(), v.end(), m) == v.end())
+    v.push_back(m);
+}
+
+bool Queue::Decode(const VkCommandBuffer_T *cb, Submission &sub, std::string &err)
+{
+  const uint8_t *cur = cb->stream.data(), *end = cur + cb->stream.size();
+  // bound state (reference gpu.h:3-23), reset per command buffer like cmd_exec.cpp:20
+  VkPipeline_T *pipeline = nullptr;
+  VkDescriptorSet_T *sets[8] = {};
+  uint8_t push[128] = {};
+  VkBuffer_T *ib = nullptr;
+  VkDeviceSize ib_off = 0;
+  VkIndexType ib_type = VK_INDEX_TYPE_UINT16;
+  VkBuffer_T *vbs[4] = {};
+  VkDeviceSize vb_off[4] = {};
+  int pass = -1;
+  auto pull = [&](size_t n) {
+    const uint8_t *p = cur;
+    cur += n;
+    return p;
+  };
+  while(cur < end)
+  {
+    Cmd id;
+    memcpy(&id, pull(2), 2);
+    switch(id)
+    {
+      case Cmd::PipelineBarrier: case Cmd::SetScissors: break;
+      case Cmd::EndRenderPass: pass = -1; break;
+      case Cmd::BeginRenderPass:
+      {
+        const cmd::BeginRenderPass *d = (const cmd::BeginRenderPass *)pull(sizeof(cmd::BeginRenderPass));
+        VkRenderPass_T *rp = d->renderPass;
+        if(!rp || rp->subpasses.empty() || rp->subpasses[0].colAttachments.empty())
+        {
+          err = "render pass without a colour attachment in subpass 0";
+          return false;
+        }
+        Op op;
+        op.kind = Op::Pass;
+        // reference cmd_exec.cpp:35-64: subpass 0, colour attachment 0, depth, clears in order
+        const auto &sp = rp->subpasses[0];
+        const auto &c0 = sp.colAttachments[0];
+        if(!d->framebuffer || c0.idx < 0 || (size_t)c0.idx >= d->framebuffer->attachments.size())
+        {
+          err = "framebuffer has no colour attachment at the render pass's index";
+          return false;
+        }
+        op.frame.color = d->framebuffer->attachments[c0.idx]->image;
+        if(!op.frame.color->mem)
+        {
+          err = "colour attachment has no memory bound";
+          return false;
+        }
+        int ci = 0;
+        if(c0.clear)
+        {
+          op.frame.clear_color = true;
+          op.frame.clear_col = d->clearval[ci++];
+        }
+        if(sp.depthAttachment.idx >= 0)
+        {
+          if((size_t)sp.depthAttachment.idx >= d->framebuffer->attachments.size() ||
+             !d->framebuffer->attachments[sp.depthAttachment.idx]->image->mem)
+          {
+            err = "depth attachment missing or without memory";
+            return false;
+          }
+          op.frame.depth = d->framebuffer->attachments[sp.depthAttachment.idx]->image;
+          if(sp.depthAttachment.clear)
+          {
+            op.frame.clear_depth = true;
+            op.frame.clear_dep = d->clearval[ci++];
+          }
+        }
+        AddMem(sub.uploads, op.frame.color->mem);
+        AddMem(sub.downloads, op.frame.color->mem);
+        if(op.frame.depth)
+        {
+          AddMem(sub.uploads, op.frame.depth->mem);
+          AddMem(sub.downloads, op.frame.depth->mem);
+        }
+        sub.ops.push_back(std::move(op));
+        pass = (int)sub.ops.size() - 1;
+        break;
+      }
+      case Cmd::BindPipeline: pipeline = ((const cmd::BindPipeline *)pull(sizeof(cmd::BindPipeline)))->pipeline; break;
+      case Cmd::BindDescriptorSets:
+      {
+        const cmd::BindDescriptorSets *d = (const cmd::BindDescriptorSets *)pull(sizeof(cmd::BindDescriptorSets));
+        if(d->idx < 8)
+          sets[d->idx] = d->set;
+        break;
+      }
+      case Cmd::BindVB:
+      {
+        const cmd::BindVB *d = (const cmd::BindVB *)pull(sizeof(cmd::BindVB));
+        if(d->slot < 4)
+        {
+          vbs[d->slot] = d->buffer;
+          vb_off[d->slot] = d->offset;
+        }
+        break;
+      }
+      case Cmd::BindIB:
+      {
+        const cmd::BindIB *d = (const cmd::BindIB *)pull(sizeof(cmd::BindIB));
+        ib = d->buffer;
+        ib_off = d->offset;
+        ib_type = d->indexType;
+        break;
+      }
+      case Cmd::SetViewport: pull(sizeof(cmd::SetViewport)); break;  // recorded, ignored (reference :108-114)
+      case Cmd::PushConstants:
+      {
+        const cmd::PushConstants *d = (const cmd::PushConstants *)pull(sizeof(cmd::PushConstants));
+        if(d->offset < 128)
+          memcpy(push + d->offset, d->values, std::min<uint32_t>(d->size, 128 - d->offset));
+        break;
+      }
+      case Cmd::Draw: case Cmd::DrawIndexed:
+      {
+        uint32_t count, first;
+        bool indexed = id == Cmd::DrawIndexed;
+        if(indexed)
+        {
+          const cmd::DrawIndexed *d = (const cmd::DrawIndexed *)pull(sizeof(cmd::DrawIndexed));
+          count = d->indexCount;
+          first = d->firstIndex;
+        }
+        else
+        {
+          const cmd::Draw *d = (const cmd::Draw *)pull(sizeof(cmd::Draw));
+          count = d->vertexCount;
+          first = d->firstVertex;
+        }
+        if(pass < 0)
+        {
+          err = "draw outside a render pass";
+          return false;
+        }
+        if(!pipeline || !pipeline->pair)
+        {
+          err = "draw without a graphics pipeline";
+          return false;
+        }
+        DrawDesc dd;
+        memset(&dd.d, 0, sizeof(dd.d));
+        dd.pipeline = pipeline;
+        dd.frame = (uint32_t)pass;
+        VaDraw &v = dd.d;
+        // ShadeVerts (rasterizer.cpp:121-237): whole triangles of a list, n-2 of a strip
+        if(pipeline->topology == VK_PRIMITIVE_TOPOLOGY_TRIANGLE_LIST)
+          v.ntris = count / 3;
+        else if(pipeline->topology == VK_PRIMITIVE_TOPOLOGY_TRIANGLE_STRIP)
+        {
+          v.ntris = count >= 3 ? count - 2 : 0;
+          v.flags |= VA_DRAW_STRIP;
+        }
+        else
+          v.ntris = 0;  // the reference prints "Unsupported primitive topology!" and draws nothing
+        v.first = first;
+        if(indexed)
+        {
+          if(!ib || !ib->mem)
+          {
+            err = "indexed draw without an index buffer";
+            return false;
+          }
+          v.flags |= VA_DRAW_INDEXED | (ib_type == VK_INDEX_TYPE_UINT32 ? VA_DRAW_INDEX32 : 0u);
+          v.ib = (unsigned long long)(uintptr_t)(ib->mem->dev + ib->offset + ib_off);
+          v.ib_bytes = BoundBytes(ib, ib_off);
+          AddMem(sub.uploads, ib->mem);
+        }
+        for(int s = 0; s < 4; s++)
+          if(vbs[s] && vbs[s]->mem)
+          {
+            v.vb[s] = (unsigned long long)(uintptr_t)(vbs[s]->mem->dev + vbs[s]->offset + vb_off[s]);
+            v.vb_bytes[s] = BoundBytes(vbs[s], vb_off[s]);
+            AddMem(sub.uploads, vbs[s]->mem);
+          }
+        for(int a = 0; a < 16; a++)
+        {
+          v.vattr_fmt[a] = pipeline->vattrs[a].format;
+          v.vattr_stride[a] = pipeline->vattrs[a].stride;
+          v.vattr_off[a] = pipeline->vattrs[a].offset;
+          v.vattr_vb[a] = std::min(pipeline->vattrs[a].vb, 3u);
+        }
+        v.cull_mode = pipeline->cullMode;
+        v.front_face = pipeline->frontFace;
+        v.depth_op = pipeline->depthCompareOp;
+        v.depth_write = pipeline->depthWriteEnable ? 1u : 0u;
+        v.blend_enable = pipeline->blend.blendEnable ? 1u : 0u;
+        v.src_factor = pipeline->blend.srcColorBlendFactor;
+        v.dst_factor = pipeline->blend.dstColorBlendFactor;
+        v.blend_op = pipeline->blend.colorBlendOp;
+        memcpy(v.push, push, 128);
+        for(const ShaderResource &r : pipeline->pair->resources)
+        {
+          VkDescriptorSet_T *ds = r.set < 8 ? sets[r.set] : nullptr;
+          if(!ds || r.binding >= ds->binds.size())
+          {
+            err = "draw references an unbound descriptor";
+            return false;
+          }
+          const auto &b = ds->binds[r.binding];
+          if(r.image)
+          {
+            if(!b.imageInfo.imageView || !b.imageInfo.imageView->image)
+            {
+              err = "combined image sampler descriptor without an image view";
+              return false;
+            }
+            VkImage_T *img = b.imageInfo.imageView->image;
+            if(!img->mem)
+            {
+              err = "sampled image has no memory bound";
+              return false;
+            }
+            FillTex(v.tex[r.binding], img, b.imageInfo.sampler);
+            AddMem(sub.uploads, img->mem);
+          }
+          else
+          {
+            VkBuffer_T *buf = b.bufferInfo.buffer;
+            if(!buf || !buf->mem)
+            {
+              err = "uniform buffer descriptor without a buffer";
+              return false;
+            }
+            v.ubo[r.binding] = (unsigned long long)(uintptr_t)(buf->mem->dev + buf->offset + b.bufferInfo.offset);
+            v.ubo_bytes[r.binding] = BoundBytes(buf, b.bufferInfo.offset);
+            AddMem(sub.uploads, buf->mem);
+          }
+        }
+        if(v.ntris)
+          sub.ops[pass].draws.push_back(dd);
+        break;
+      }
+      case Cmd::CopyBuf2Img:
+      {
+        const cmd::CopyBuf2Img *d = (const cmd::CopyBuf2Img *)pull(sizeof(cmd::CopyBuf2Img));
+        Op op;
+        op.kind = Op::Copy;
+        op.copy.to_image = true;
+        op.copy.src = d->srcBuffer;
+        op.copy.dstImg = d->dstImage;
+        op.copy.dstBuf = nullptr;
+        op.copy.bi = d->region;
+        AddMem(sub.uploads, d->srcBuffer->mem);
+        AddMem(sub.downloads, d->dstImage->mem);
+        sub.ops.push_back(op);
+        break;
+      }
+      case Cmd::CopyBuf:
+      {
+        const cmd::CopyBuf *d = (const cmd::CopyBuf *)pull(sizeof(cmd::CopyBuf));
+        Op op;
+        op.kind = Op::Copy;
+        op.copy.to_image = false;
+        op.copy.src = d->srcBuffer;
+        op.copy.dstBuf = d->dstBuffer;
+        op.copy.dstImg = nullptr;
+        op.copy.bb = d->region;
+        AddMem(sub.uploads, d->srcBuffer->mem);
+        AddMem(sub.downloads, d->dstBuffer->mem);
+        sub.ops.push_back(op);
+        break;
+      }
+      default:
+        err = "corrupt command stream";
+        return false;
+    }
+  }
+  return true;
+}
+
+VkResult Queue::RunCopy(const CopyDesc &c)
+{
+  if(c.to_image)
+  {
+    // reference cmd_exec.cpp:143-173: whole mip, tightly packed, one layer
+    const VkImage_T *img = c.dstImg;
+    uint32_t mip = c.bi.imageSubresource.mipLevel;
+    uint64_t w = std::max(1u, img->extent.width >> mip), h = std::max(1u, img->extent.height >> mip);
+    if(c.bi.bufferRowLength != 0 || c.bi.bufferImageHeight != 0 || c.bi.imageOffset.x || c.bi.imageOffset.y ||
+       c.bi.imageOffset.z || c.bi.imageExtent.width != w || c.bi.imageExtent.height != h ||
+       c.bi.imageSubresource.layerCount != 1)
+    {
+      LogError("vkCmdCopyBufferToImage: only whole, tightly packed single-layer mips are supported");
+      return VK_ERROR_FEATURE_NOT_PRESENT;
+    }
+    uint64_t offs = SubresourceOffset(img, mip, c.bi.imageSubresource.baseArrayLayer);
+    HIPCHK(hipMemcpyAsync(img->mem->dev + img->offset + offs, c.src->mem->dev + c.src->offset + c.bi.bufferOffset,
+                          w * h * img->bytesPerPixel, hipMemcpyDeviceToDevice, stream));
+  }
+  else
+  {
+    HIPCHK(hipMemcpyAsync(c.dstBuf->mem->dev + c.dstBuf->offset + c.bb.dstOffset,
+                          c.src->mem->dev + c.src->offset + c.bb.srcOffset, c.bb.size, hipMemcpyDeviceToDevice, stream));
+  }
+  return VK_SUCCESS;
+}
+
+// reference rasterizer.cpp:312-361 (ClearTarget x2)
+VkResult Queue::ClearFrame(const FrameDesc &f)
+{
+  if(f.clear_color)
+  {
+    VkImage_T *img = f.color;
+    uint64_t n = (uint64_t)img->extent.width * img->extent.height;
+    const float *c = f.clear_col.color.float32;
+    if(img->bytesPerPixel == 8)
+    {
+      // RGBA16F (beyond the reference)
+      uint32_t h[4];
+      for(int i = 0; i < 4; i++)
+        h[i] = HalfBits(c[i]);
+      HIPCHK(va_fill_u64_async(img->devPixels(), n, h[0] | (h[1] << 16), h[2] | (h[3] << 16), stream));
+    }
+    else
+    {
+      uint8_t e[4];
+      e[2] = (uint8_t)(int)(c[0] * 255.0f);
+      e[1] = (uint8_t)(int)(c[1] * 255.0f);
+      e[0] = (uint8_t)(int)(c[2] * 255.0f);
+      e[3] = (uint8_t)(int)(c[3] * 255.0f);
+      if(img->bytesPerPixel == 1)
+        HIPCHK(hipMemsetD8Async((hipDeviceptr_t)img->devPixels(), e[2], n, stream));
+      else
+      {
+        uint32_t v;
+        memcpy(&v, e, 4);
+        HIPCHK(va_fill_u32_async(img->devPixels(), n, v, stream));
+      }
+    }
+  }
+  if(f.clear_depth && f.depth)
+  {
+    uint64_t n = (uint64_t)f.depth->extent.width * f.depth->extent.height;
+    uint32_t v;
+    memcpy(&v, &f.clear_dep.depthStencil.depth, 4);
+    HIPCHK(va_fill_u32_async(f.depth->devPixels(), n, v, stream));
+  }
+  return VK_SUCCESS;
+}
+
+static int BitsFor(uint32_t n)
+{
+  int b = 1;
+  while(b < 32 && (1ull << b) < n)
+    b++;
+  return b;
+}
+
+VkResult Queue::RunPass(const std::vector<Op *> &passes, bool fused_clear, ExecStats &st)
+{
+  const uint32_t ts = tiles.tile_shift, T = 1u << ts;
+  std::vector<VaFrame> frames;
+  std::vector<VaDraw> draws;
+  std::vector<std::shared_ptr<ShaderPair>> pairs;
+  uint32_t tile_total = 0, tri_total = 0;
+  for(Op *p : passes)
+  {
+    VaFrame fr;
+    memset(&fr, 0, sizeof(fr));
+    VkImage_T *c = p->frame.color;
+    fr.color = (unsigned long long)(uintptr_t)c->devPixels();
+    fr.depth = p->frame.depth ? (unsigned long long)(uintptr_t)p->frame.depth->devPixels() : 0ull;
+    fr.width = c->extent.width;
+    fr.height = c->extent.height;
+    fr.color_bpp = c->bytesPerPixel;
+    if(fused_clear)
+    {
+      fr.clear_flags = (p->frame.clear_color ? VA_CLEAR_COLOR : 0u) | (p->frame.clear_depth && p->frame.depth ? VA_CLEAR_DEPTH : 0u);
+      memcpy(fr.clear_color, p->frame.clear_col.color.float32, 16);
+      fr.clear_depth = p->frame.clear_dep.depthStencil.depth;
+    }
+    fr.tiles_x = (fr.width + T - 1) >> ts;
+    fr.tiles_y = (fr.height + T - 1) >> ts;
+    fr.tile_base = tile_total;
+    if(fr.width >= 65535 || fr.height >= 65535)
+    {
+      LogError("render target larger than 65534 pixels per side");
+      return VK_ERROR_FEATURE_NOT_PRESENT;
+    }
+    tile_total += fr.tiles_x * fr.tiles_y;
+    for(DrawDesc &dd : p->draws)
+    {
+      VaDraw v = dd.d;
+      v.frame = (uint32_t)frames.size();
+      auto it = std::find(pairs.begin(), pairs.end(), dd.pipeline->pair);
+      if(it == pairs.end())
+      {
+        pairs.push_back(dd.pipeline->pair);
+        it = pairs.end() - 1;
+      }
+      v.slot = (uint32_t)(it - pairs.begin());
+      v.tri_base = tri_total;
+      tri_total += v.ntris;
+      draws.push_back(v);
+    }
+    frames.push_back(fr);
+  }
+  bool any_clear = false;
+  for(auto &f : frames)
+    any_clear |= f.clear_flags != 0;
+  if(tri_total == 0 && !any_clear)
+    return VK_SUCCESS;
+  std::shared_ptr<PipelineModule> mod;
+  if(!pairs.empty())
+  {
+    std::string err;
+    mod = Device::Get().GetModule(pairs, err);
+    if(!mod)
+    {
+      LogError("pipeline module: %s", err.c_str());
+      return VK_ERROR_DEVICE_LOST;
+    }
+  }
+  else
+  {
+    // a clear-only pass still needs raster kernels: any module will do; use a cached one
+    LogError("clear-only render pass in batched mode: using fill kernels");
+    for(Op *p : passes)
+    {
+      VkResult r = ClearFrame(p->frame);
+      if(r != VK_SUCCESS)
+        return r;
+    }
+    return VK_SUCCESS;
+  }
+  const uint32_t stride = mod->rec_stride;
+  const uint32_t nframes = (uint32_t)frames.size(), ndraws = (uint32_t)draws.size();
+  if(!Grow(&d_frames, c_frames, sizeof(VaFrame) * nframes) || !Grow(&d_draws, c_draws, sizeof(VaDraw) * std::max(ndraws, 1u)) ||
+     !Grow(&d_rec, c_rec, (size_t)stride * 4 * std::max(tri_total, 1u)) || !Grow(&d_count, c_count, 4ull * (tri_total + 1)) ||
+     !Grow(&d_offs, c_offs, 4ull * (tri_total + 1)) || !Grow(&d_start, c_start, 4ull * tile_total) ||
+     !Grow(&d_end, c_end, 4ull * tile_total) || !Grow(&d_ctr, c_ctr, sizeof(VaCounters)))
+    return VK_ERROR_OUT_OF_DEVICE_MEMORY;
+  if(!h_pinned)
+    HIPCHK(hipHostMalloc((void **)&h_pinned, 4096, hipHostMallocDefault));
+  for(int i = 0; i < 8; i++)
+    if(!ev[i])
+      HIPCHK(hipEventCreate(&ev[i]));
+
+  HIPCHK(hipMemcpyAsync(d_frames, frames.data(), sizeof(VaFrame) * nframes, hipMemcpyHostToDevice, stream));
+  if(ndraws)
+    HIPCHK(hipMemcpyAsync(d_draws, draws.data(), sizeof(VaDraw) * ndraws, hipMemcpyHostToDevice, stream));
+  HIPCHK(va_fill_u32_async(d_ctr, sizeof(VaCounters) / 4, 0u, stream));
+  HIPCHK(hipEventRecord(ev[0], stream));
+  uint32_t total_pairs = 0;
+  if(tri_total)
+  {
+    VaDraw *dd = (VaDraw *)d_draws;
+    VaFrame *df = (VaFrame *)d_frames;
+    uint32_t *rec = (uint32_t *)d_rec, *cnt = (uint32_t *)d_count;
+    VaCounters *ctr = (VaCounters *)d_ctr;
+    void *args[] = {&dd, (void *)&ndraws, &df, &tri_total, (void *)&ts, &rec, &cnt, &ctr};
+    HIPCHK(hipModuleLaunchKernel(mod->setup, (tri_total + 255) / 256, 1, 1, 256, 1, 1, 0, stream, args, nullptr));
+    HIPCHK(va_fill_u32_async(cnt + tri_total, 1, 0u, stream));
+    size_t need = 0;
+    HIPCHK(va_scan_u32(nullptr, &need, cnt, (uint32_t *)d_offs, tri_total + 1, stream));
+    if(!Grow(&d_temp, c_temp, need))
+      return VK_ERROR_OUT_OF_DEVICE_MEMORY;
+    size_t have = c_temp;
+    HIPCHK(va_scan_u32(d_temp, &have, cnt, (uint32_t *)d_offs, tri_total + 1, stream));
+    HIPCHK(hipMemcpyAsync(h_pinned, (uint32_t *)d_offs + tri_total, 4, hipMemcpyDeviceToHost, stream));
+    HIPCHK(hipEventRecord(ev[1], stream));
+    HIPCHK(hipStreamSynchronize(stream));
+    total_pairs = h_pinned[0];
+  }
+  HIPCHK(va_fill_u32_async(d_start, tile_total, 0u, stream));
+  HIPCHK(va_fill_u32_async(d_end, tile_total, 0u, stream));
+  if(total_pairs)
+  {
+    if(!Grow(&d_keys, c_keys, 4ull * total_pairs) || !Grow(&d_vals, c_vals, 4ull * total_pairs) ||
+       !Grow(&d_keys2, c_keys2, 4ull * total_pairs) || !Grow(&d_vals2, c_vals2, 4ull * total_pairs))
+      return VK_ERROR_OUT_OF_DEVICE_MEMORY;
+    uint32_t *rec = (uint32_t *)d_rec, *offs = (uint32_t *)d_offs, *keys = (uint32_t *)d_keys, *vals = (uint32_t *)d_vals;
+    VaDraw *dd = (VaDraw *)d_draws;
+    VaFrame *df = (VaFrame *)d_frames;
+    void *args[] = {&rec, &offs, &tri_total, &total_pairs, &dd, &df, (void *)&ts, &keys, &vals};
+    HIPCHK(hipModuleLaunchKernel(mod->emit, (total_pairs + 255) / 256, 1, 1, 256, 1, 1, 0, stream, args, nullptr));
+    int bits = BitsFor(tile_total);
+    size_t need = 0;
+    HIPCHK(va_sort_pairs(nullptr, &need, keys, (uint32_t *)d_keys2, vals, (uint32_t *)d_vals2, total_pairs, bits, stream));
+    if(!Grow(&d_temp, c_temp, need))
+      return VK_ERROR_OUT_OF_DEVICE_MEMORY;
+    size_t have = c_temp;
+    HIPCHK(va_sort_pairs(d_temp, &have, keys, (uint32_t *)d_keys2, vals, (uint32_t *)d_vals2, total_pairs, bits, stream));
+    uint32_t *sk = (uint32_t *)d_keys2, *st_ = (uint32_t *)d_start, *en = (uint32_t *)d_end;
+    void *rargs[] = {&sk, &total_pairs, &st_, &en};
+    HIPCHK(hipModuleLaunchKernel(mod->ranges, (total_pairs + 255) / 256, 1, 1, 256, 1, 1, 0, stream, rargs, nullptr));
+  }
+  {
+    HIPCHK(hipEventRecord(ev[2], stream));
+    VaDraw *dd = (VaDraw *)d_draws;
+    VaFrame *df = (VaFrame *)d_frames;
+    uint32_t *rec = (uint32_t *)d_rec, *bins = (uint32_t *)d_vals2, *st_ = (uint32_t *)d_start, *en = (uint32_t *)d_end;
+    VaCounters *ctr = (VaCounters *)d_ctr;
+    uint32_t use_fused = fused_clear ? 1u : 0u;
+    void *args[] = {&dd, &df, (void *)&nframes, &rec, &bins, &st_, &en, &tile_total, &use_fused, &ctr};
+    hipFunction_t k = mod->raster[ts - 3];
+    uint32_t threads = (T / 8) * (T / 8) * 64;
+    HIPCHK(hipModuleLaunchKernel(k, tile_total, 1, 1, threads, 1, 1, 0, stream, args, nullptr));
+    HIPCHK(hipEventRecord(ev[3], stream));
+  }
+  HIPCHK(hipMemcpyAsync(h_pinned + 64, d_ctr, sizeof(VaCounters), hipMemcpyDeviceToHost, stream));
+  HIPCHK(hipStreamSynchronize(stream));
+  VaCounters c;
+  memcpy(&c, h_pinned + 64, sizeof(c));
+  c.draws = ndraws;
+  float ms = 0.0f;
+  if(tri_total)
+  {
+    hipEventElapsedTime(&ms, ev[0], ev[1]);
+    st.setup_ms += ms;
+    hipEventElapsedTime(&ms, ev[1], ev[2]);
+    st.bin_ms += ms;
+  }
+  hipEventElapsedTime(&ms, ev[2], ev[3]);
+  st.raster_ms += ms;
+  st.raster_launches++;
+  st.total_tris += tri_total;
+  st.total_pairs += total_pairs;
+  st.total_tiles += tile_total;
+  st.ctr.draws += c.draws;
+  st.ctr.tris_in += c.tris_in;
+  st.ctr.tris_out += c.tris_out;
+  st.ctr.blocks += c.blocks;
+  st.ctr.pixels_tested += c.pixels_tested;
+  st.ctr.pixels_written += c.pixels_written;
+  st.ctr.depth_passed += c.depth_passed;
+  return VK_SUCCESS;
+}
+
+void Queue::SignalFence(VkFence f, VkResult r)
+{
+  if(!f)
+    return;
+  std::lock_guard<std::mutex> lk(f->m);
+  f->signaled = true;
+  f->status = r;
+  f->cv.notify_all();
+}
+
+VkResult Queue::Execute(std::vector<Submission> &subs)
+{
+  std::lock_guard<std::mutex> lk(submit_m);
+  if(!stream)
+    HIPCHK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+  VkResult res = VK_SUCCESS;
+  for(Submission &sub : subs)
+  {
+    ExecStats st;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    for(VkDeviceMemory_T *m : sub.uploads)
+      HIPCHK(hipMemcpyAsync(m->dev, m->host, m->size, hipMemcpyHostToDevice, stream));
+    HIPCHK(hipEventRecord(e0, stream));
+    for(Op &op : sub.ops)
+    {
+      if(res != VK_SUCCESS)
+        break;
+      if(op.kind == Op::Copy)
+        res = RunCopy(op.copy);
+      else if(mode == ExecMode::Batched)
+        res = RunPass({&op}, true, st);
+      else
+      {
+        res = ClearFrame(op.frame);
+        for(size_t i = 0; i < op.draws.size() && res == VK_SUCCESS; i++)
+        {
+          Op one;
+          one.kind = Op::Pass;
+          one.frame = op.frame;
+          one.draws.assign(1, op.draws[i]);
+          res = RunPass({&one}, false, st);
+        }
+      }
+    }
+    HIPCHK(hipEventRecord(e1, stream));
+    for(VkDeviceMemory_T *m : sub.downloads)
+      HIPCHK(hipMemcpyAsync(m->host, m->dev, m->size, hipMemcpyDeviceToHost, stream));
+    hipError_t e = hipStreamSynchronize(stream);
+    if(e != hipSuccess && res == VK_SUCCESS)
+      res = HipToVk(e);
+    float ms = 0.0f;
+    hipEventElapsedTime(&ms, e0, e1);
+    st.gpu_ms_total = ms;
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    last_stats = st;
+    accum_stats.gpu_ms_total += st.gpu_ms_total;
+    accum_stats.raster_ms += st.raster_ms;
+    accum_stats.setup_ms += st.setup_ms;
+    accum_stats.bin_ms += st.bin_ms;
+    accum_stats.raster_launches += st.raster_launches;
+    accum_stats.total_tris += st.total_tris;
+    accum_stats.total_pairs += st.total_pairs;
+    accum_stats.total_tiles += st.total_tiles;
+    accum_stats.ctr.pixels_written += st.ctr.pixels_written;
+    accum_stats.ctr.depth_passed += st.ctr.depth_passed;
+    accum_stats.ctr.tris_in += st.ctr.tris_in;
+    SignalFence(sub.fence, res);
+  }
+  return res;
+}
+
+void Queue::WorkerLoop()
+{
+  for(;;)
+  {
+    std::vector<Submission> batch;
+    {
+      std::unique_lock<std::mutex> lk(qm);
+      qcv.wait(lk, [&] { return stop || !pending.empty(); });
+      if(stop && pending.empty())
+        return;
+      while(!pending.empty())
+      {
+        batch.push_back(std::move(pending.front()));
+        pending.pop_front();
+      }
+      busy = true;
+    }
+    Execute(batch);
+    {
+      std::lock_guard<std::mutex> lk(qm);
+      busy = false;
+    }
+    idle_cv.notify_all();
+  }
+}
+
+VkResult Queue::Submit(uint32_t count, const VkSubmitInfo *infos, VkFence fence)
+{
+  if(!Device::Get().ok())
+    return VK_ERROR_DEVICE_LOST;
+  std::vector<Submission> subs;
+  for(uint32_t i = 0; i < count; i++)
+    for(uint32_t c = 0; c < infos[i].commandBufferCount; c++)
+    {
+      Submission s;
+      std::string err;
+      if(!Decode(infos[i].pCommandBuffers[c], s, err))
+      {
+        LogError("vkQueueSubmit: %s", err.c_str());
+        return VK_ERROR_FEATURE_NOT_PRESENT;
+      }
+      s.t_submit = NowSeconds();
+      subs.push_back(std::move(s));
+    }
+  if(fence)
+  {
+    std::lock_guard<std::mutex> lk(fence->m);
+    fence->signaled = false;
+  }
+  if(subs.empty())
+  {
+    SignalFence(fence, VK_SUCCESS);
+    return VK_SUCCESS;
+  }
+  subs.back().fence = fence;
+  if(!async)
+    return Execute(subs);  // synchronous, as the reference's vkQueueSubmit (cmd_exec.cpp:187-201)
+  {
+    std::lock_guard<std::mutex> lk(qm);
+    if(!worker.joinable())
+      worker = std::thread([this] { WorkerLoop(); });
+    for(auto &s : subs)
+      pending.push_back(std::move(s));
+  }
+  qcv.notify_all();
+  return VK_SUCCESS;
+}
+
+VkResult Queue::WaitIdle()
+{
+  std::unique_lock<std::mutex> lk(qm);
+  idle_cv.wait(lk, [&] { return pending.empty() && !busy; });
+  return VK_SUCCESS;
+}
+}  // namespace visor

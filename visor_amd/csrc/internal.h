@@ -1,0 +1,406 @@
+// internal.h - the ICD's object model and runtime (host side).
+//
+// Object model follows the reference (precompiled.h:22-203): dispatchable objects start with the
+// loader magic word; non-dispatchable ones are plain structs created with new. Memory objects are
+// backed by HBM; host-visible ones also keep a pinned host mirror that the application maps.
+#pragma once
+
+#include <hip/hip_runtime_api.h>
+
+#include <atomic>
+#include <condition_variable>
+#include <cstdint>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "../../include/visor_vk.h"
+#include "device_abi.h"
+#include "spirv_hip.h"
+
+namespace visor
+{
+void SetLoaderMagic(void *obj);
+const char *HipErrorName(hipError_t e);
+VkResult HipToVk(hipError_t e);
+void LogError(const char *fmt, ...);
+uint16_t HalfBits(float f);  // binary32 -> binary16, round to nearest even
+
+// ---------------------------------------------------------------------------------------------
+// command stream: {uint16 id}{POD}, as the reference's VkCommandBuffer_T::push (precompiled.h:63-69)
+enum class Cmd : uint16_t
+{
+  PipelineBarrier,
+  BeginRenderPass,
+  EndRenderPass,
+  BindPipeline,
+  BindDescriptorSets,
+  BindVB,
+  BindIB,
+  SetViewport,
+  SetScissors,
+  PushConstants,
+  Draw,
+  DrawIndexed,
+  CopyBuf2Img,
+  CopyBuf,
+};
+
+namespace cmd
+{
+struct BeginRenderPass { VkRenderPass renderPass; VkFramebuffer framebuffer; VkClearValue clearval[8]; };
+struct BindPipeline { VkPipeline pipeline; };
+struct BindDescriptorSets { uint32_t idx; VkDescriptorSet set; };
+struct BindVB { uint32_t slot; VkBuffer buffer; VkDeviceSize offset; };
+struct BindIB { VkBuffer buffer; VkDeviceSize offset; VkIndexType indexType; };
+struct SetViewport { VkViewport view; };
+struct PushConstants { uint32_t offset, size; uint8_t values[128]; };
+struct Draw { uint32_t vertexCount, instanceCount, firstVertex, firstInstance; };
+struct DrawIndexed { uint32_t indexCount, instanceCount, firstIndex, firstInstance; int32_t vertexOffset; };
+struct CopyBuf2Img { VkBuffer srcBuffer; VkImage dstImage; VkBufferImageCopy region; };
+struct CopyBuf { VkBuffer srcBuffer; VkBuffer dstBuffer; VkBufferCopy region; };
+}  // namespace cmd
+
+class Runtime;
+class Queue;
+struct PipelineModule;
+struct ShaderPair;
+}  // namespace visor
+
+struct VkInstance_T
+{
+  uintptr_t loaderMagic;
+};
+
+struct VkPhysicalDevice_T
+{
+  uintptr_t loaderMagic;
+};
+
+struct VkQueue_T
+{
+  uintptr_t loaderMagic;
+  visor::Queue *q;
+};
+
+struct VkDevice_T
+{
+  uintptr_t loaderMagic;
+  std::vector<VkQueue_T *> queues;
+};
+
+struct VkDeviceMemory_T
+{
+  VkDeviceSize size = 0;
+  uint32_t type = 0;          // 0 DEVICE_LOCAL, 1 HOST_VISIBLE|COHERENT|CACHED
+  uint8_t *host = nullptr;    // pinned mirror (type 1)
+  uint8_t *dev = nullptr;     // HBM
+};
+
+struct VkBuffer_T
+{
+  VkDeviceSize size = 0;
+  VkDeviceMemory_T *mem = nullptr;
+  VkDeviceSize offset = 0;
+};
+
+struct VkImage_T
+{
+  VkExtent3D extent = {0, 0, 0};
+  VkImageType imageType = VK_IMAGE_TYPE_2D;
+  VkFormat format = VK_FORMAT_UNDEFINED;
+  uint32_t arrayLayers = 1;
+  uint32_t mipLevels = 1;
+  uint32_t bytesPerPixel = 4;
+  VkDeviceMemory_T *mem = nullptr;
+  VkDeviceSize offset = 0;
+  uint8_t *devPixels() const { return mem ? mem->dev + offset : nullptr; }
+};
+
+struct VkImageView_T
+{
+  VkImage_T *image = nullptr;
+};
+
+struct VkSampler_T
+{
+  VkSamplerCreateInfo info;
+};
+
+struct VkShaderModule_T
+{
+  std::vector<uint32_t> code;
+};
+
+struct VkPipeline_T
+{
+  struct
+  {
+    VkFormat format;
+    uint32_t stride;
+    uint32_t offset;
+    uint32_t vb;
+  } vattrs[16];
+  VkPrimitiveTopology topology = VK_PRIMITIVE_TOPOLOGY_MAX_ENUM;
+  VkFrontFace frontFace = VK_FRONT_FACE_COUNTER_CLOCKWISE;
+  VkCullModeFlags cullMode = VK_CULL_MODE_NONE;
+  VkCompareOp depthCompareOp = VK_COMPARE_OP_ALWAYS;
+  bool depthWriteEnable = false;
+  VkPipelineColorBlendAttachmentState blend;
+  std::shared_ptr<visor::ShaderPair> pair;
+  bool graphics = true;
+};
+
+struct VkDescriptorSetLayout_T
+{
+  uint32_t bindingCount = 0;
+};
+
+struct VkDescriptorSet_T
+{
+  struct Bind
+  {
+    VkDescriptorType type = VK_DESCRIPTOR_TYPE_MAX_ENUM;
+    VkDescriptorImageInfo imageInfo;
+    VkDescriptorBufferInfo bufferInfo;
+  };
+  std::vector<Bind> binds;
+};
+
+struct VkRenderPass_T
+{
+  struct Attachment
+  {
+    int32_t idx = -1;
+    bool clear = false;
+  };
+  struct Subpass
+  {
+    std::vector<Attachment> colAttachments;
+    Attachment depthAttachment;
+  };
+  std::vector<Subpass> subpasses;
+};
+
+struct VkFramebuffer_T
+{
+  std::vector<VkImageView> attachments;
+};
+
+struct VkCommandBuffer_T
+{
+  uintptr_t loaderMagic;
+  bool live = false;
+  std::vector<uint8_t> stream;
+
+  template <typename T>
+  T *push(visor::Cmd id)
+  {
+    size_t at = stream.size();
+    stream.resize(at + sizeof(uint16_t) + sizeof(T));
+    memcpy(&stream[at], &id, sizeof(uint16_t));
+    T *ret = (T *)&stream[at + sizeof(uint16_t)];
+    memset((void *)ret, 0, sizeof(T));
+    return ret;
+  }
+  void pushId(visor::Cmd id)
+  {
+    size_t at = stream.size();
+    stream.resize(at + sizeof(uint16_t));
+    memcpy(&stream[at], &id, sizeof(uint16_t));
+  }
+};
+
+struct VkCommandPool_T
+{
+  std::vector<VkCommandBuffer> buffers;
+};
+
+struct VkFence_T
+{
+  std::mutex m;
+  std::condition_variable cv;
+  bool signaled = false;
+  VkResult status = VK_SUCCESS;
+};
+
+struct VkSemaphore_T
+{
+  int dummy = 0;
+};
+
+struct VkSwapchainKHR_T
+{
+  VkExtent2D extent = {0, 0};
+  VkFormat format = VK_FORMAT_B8G8R8A8_UNORM;
+  std::vector<VkImage> images;
+  std::vector<VkDeviceMemory> mems;
+  uint32_t current = 0;
+  uint64_t presents = 0;
+};
+
+namespace visor
+{
+// ---------------------------------------------------------------------------------------------
+// Shader pairs and pipeline modules (hiprtc-built)
+
+struct ShaderPair
+{
+  std::string key;  // hash of both SPIR-V modules and entry names
+  TranslatedShader vs, fs;
+  uint32_t nv = 0;                 // varying components
+  std::vector<uint32_t> vary_src;  // per varying: index into the VS output array (loc*4+comp)
+  std::vector<ShaderResource> resources;
+};
+
+struct PipelineModule
+{
+  std::string key;
+  hipModule_t module = nullptr;
+  hipFunction_t setup = nullptr, emit = nullptr, ranges = nullptr;
+  hipFunction_t raster[3] = {nullptr, nullptr, nullptr};  // tile 8, 16, 32
+  uint32_t rec_stride = 0;  // words
+  std::vector<std::string> slot_keys;
+  double compile_seconds = 0.0;
+};
+
+// ---------------------------------------------------------------------------------------------
+// Executor: decoded submissions and the device work
+
+struct FrameDesc
+{
+  VkImage_T *color = nullptr;
+  VkImage_T *depth = nullptr;
+  bool clear_color = false, clear_depth = false;
+  VkClearValue clear_col{}, clear_dep{};
+};
+
+struct DrawDesc
+{
+  uint32_t frame;          // index into the pass list
+  VaDraw d;                // everything but tri_base / frame / slot
+  VkPipeline_T *pipeline;
+};
+
+struct CopyDesc
+{
+  bool to_image;
+  VkBuffer_T *src;
+  VkBuffer_T *dstBuf;
+  VkImage_T *dstImg;
+  VkBufferImageCopy bi;
+  VkBufferCopy bb;
+};
+
+struct Op
+{
+  enum Kind { Pass, Copy } kind;
+  FrameDesc frame;
+  std::vector<DrawDesc> draws;
+  CopyDesc copy;
+};
+
+struct Submission
+{
+  std::vector<Op> ops;
+  std::vector<VkDeviceMemory_T *> uploads;    // host-visible memory read by the GPU
+  std::vector<VkDeviceMemory_T *> downloads;  // host-visible memory written by the GPU
+  VkFence fence = nullptr;
+  double t_submit = 0.0;
+};
+
+struct ExecStats
+{
+  double gpu_ms_total = 0.0;    // hipEvent time around the whole submission's device work
+  double raster_ms = 0.0;       // hipEvent time around the raster kernel(s)
+  double setup_ms = 0.0;
+  double bin_ms = 0.0;
+  uint64_t raster_launches = 0;
+  uint64_t total_tris = 0, total_pairs = 0, total_tiles = 0;
+  VaCounters ctr{};
+};
+
+enum class ExecMode
+{
+  Plain = 0,    // per draw: setup, bin, raster, host sync (the reference's blocking DrawTriangles)
+  Batched = 1,  // every draw of a render pass in one setup/bin/raster sequence, fused clears
+};
+
+struct TileConfig
+{
+  uint32_t tile_shift = 4;  // 3, 4, 5 -> 8, 16, 32 pixel tiles
+};
+
+class Device
+{
+public:
+  static Device &Get();
+  bool Init(std::string &err);
+  bool ok() const { return initialized; }
+  int device_id = 0;
+  hipDeviceProp_t props{};
+  std::string arch = "gfx950";
+
+  std::shared_ptr<ShaderPair> MakePair(VkShaderModule_T *vs, const char *vs_entry, VkShaderModule_T *fs,
+                                       const char *fs_entry, std::string &err);
+  std::shared_ptr<PipelineModule> GetModule(const std::vector<std::shared_ptr<ShaderPair>> &pairs,
+                                            std::string &err);
+  std::string cache_dir;
+
+private:
+  bool initialized = false;
+  std::mutex m;
+  std::map<std::string, std::shared_ptr<ShaderPair>> pairs;
+  std::map<std::string, std::shared_ptr<PipelineModule>> modules;
+};
+
+class Queue
+{
+public:
+  Queue();
+  ~Queue();
+  VkResult Submit(uint32_t count, const VkSubmitInfo *infos, VkFence fence);
+  VkResult WaitIdle();
+  ExecStats last_stats;
+  ExecStats accum_stats;
+  ExecMode mode = ExecMode::Batched;
+  TileConfig tiles;
+  bool async = false;
+  bool Decode(const VkCommandBuffer_T *cb, Submission &sub, std::string &err);
+
+private:
+  VkResult Execute(std::vector<Submission> &subs);
+  VkResult RunPass(const std::vector<Op *> &passes, bool fused_clear, ExecStats &st);
+  VkResult RunCopy(const CopyDesc &c);
+  VkResult ClearFrame(const FrameDesc &f);
+  bool Grow(void **p, size_t &cap, size_t need);
+  void WorkerLoop();
+  void SignalFence(VkFence f, VkResult r);
+
+  hipStream_t stream = nullptr;
+  hipEvent_t ev[8] = {};
+  std::mutex submit_m;  // one submission executes at a time
+  // growable device scratch
+  void *d_frames = nullptr, *d_draws = nullptr, *d_rec = nullptr, *d_count = nullptr, *d_offs = nullptr;
+  void *d_keys = nullptr, *d_vals = nullptr, *d_keys2 = nullptr, *d_vals2 = nullptr;
+  void *d_start = nullptr, *d_end = nullptr, *d_ctr = nullptr, *d_temp = nullptr;
+  size_t c_frames = 0, c_draws = 0, c_rec = 0, c_count = 0, c_offs = 0, c_keys = 0, c_vals = 0, c_keys2 = 0,
+         c_vals2 = 0, c_start = 0, c_end = 0, c_ctr = 0, c_temp = 0;
+  uint32_t *h_pinned = nullptr;  // small pinned staging for readbacks
+
+  // async worker
+  std::thread worker;
+  std::mutex qm;
+  std::condition_variable qcv, idle_cv;
+  std::deque<Submission> pending;
+  bool stop = false;
+  bool busy = false;
+};
+
+double NowSeconds();
+}  // namespace visor
