@@ -1,0 +1,157 @@
+#!/usr/bin/env python
+"""Benchmark: batched vs plain per-draw execution of the 256-draw workload on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--tile 8|16|32]
+
+Workload (config 3): one 1920x1080 BGRA8 + D32 target, 256 draws of log-uniform 16..65536
+triangles (about 1.85M), per-draw MVP, 4 mipmapped 512x512 textures sampled trilinear repeat;
+even draws opaque (LESS), odd draws alpha-blended with depth writes off. Inputs (vertex, index,
+texture data) are resident in HBM; the target stays in HBM.
+
+A step is one vkQueueSubmit of the recorded render pass (all 256 draws), timed on the host
+between synchronisations. value = covered fragments per second in the batched path. The plain
+per-draw path (setup, bin, raster and a host wait per draw, as the reference's DrawTriangles) is
+timed beside it, and the CPU baseline is the oracle's reference-structured threaded mode on a
+bounded sample of the draws.
+"""
+
+import argparse
+import json
+import os
+import statistics
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "batched/served Gfragments/s vs plain per-draw on 1 GPU; p50/p99 frame latency"
+HBM_PEAK_GBS = 8000.0
+
+
+def raster_bytes(st, width, height, color_bytes, texel_bytes):
+    """Algorithmic HBM bytes of one raster launch (per frame): depth test reads of covered
+    fragments, depth writes, colour writes (+ reads when blending) and texels of shaded fragments,
+    and the one write of the final colour and depth of every pixel (the clear is fused)."""
+    return (4 * st["pixels_written"] + 4 * st["shaded_dwrite"] + color_bytes * st["depth_passed"]
+            + color_bytes * st["shaded_blend"] + texel_bytes * st["depth_passed"]
+            + width * height * (color_bytes + 4))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--plain-steps", type=int, default=3)
+    ap.add_argument("--tile", type=int, default=16)
+    ap.add_argument("--cpu-draws", type=int, default=128, help="draws in the CPU-baseline sample")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-plain", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: every box here has one MI355X; --gpus %d is not supported" % args.gpus,
+              file=sys.stderr)
+        sys.exit(2)
+
+    from visor_amd import render, scenes
+
+    try:
+        import torch
+        sync = torch.cuda.synchronize if torch.cuda.is_available() else (lambda: None)
+    except Exception:  # torch is plumbing only; the ICD synchronises its own stream
+        sync = lambda: None
+
+    scene = scenes.config3()
+    ctx = render.Context()
+    r = render.SceneRenderer(ctx, scene, host_visible_targets=False)
+    ctx.set_mode(batched=True, tile=args.tile)
+    cb = r.record()
+    r.cb = cb
+
+    print("[bench] config3: %d draws, %d triangles; warmup %d" % (len(scene.draws), scene.triangles, args.warmup),
+          file=sys.stderr, flush=True)
+    for _ in range(args.warmup):
+        r.render()
+    stats_one = ctx.stats(0)
+
+    # timed batched steps
+    ctx.icd.vh_reset_stats(ctx.queue)
+    lat = []
+    sync()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        lat.append(r.render())
+    sync()
+    elapsed = time.perf_counter() - t0
+    acc = ctx.stats(1)
+    frags_per_frame = stats_one["pixels_written"]
+    value = frags_per_frame * args.steps / elapsed / 1e9
+    ms_per_step = elapsed * 1e3 / args.steps
+    raster_ms = acc["raster_ms"] / max(acc["raster_launches"], 1)
+    print("[bench] batched %.3f ms/frame, raster kernel %.3f ms" % (ms_per_step, raster_ms), file=sys.stderr, flush=True)
+
+    # roofline of the dominant kernel (the raster launch, one per frame)
+    b_raster = raster_bytes(stats_one, scene.width, scene.height, 4, 32)
+    achieved = b_raster / (raster_ms * 1e-3) / 1e9
+    roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None}
+
+    # plain per-draw path
+    plain = None
+    if not args.no_plain:
+        ctx.set_mode(batched=False, tile=args.tile)
+        r.render()
+        ctx.icd.vh_reset_stats(ctx.queue)
+        t0 = time.perf_counter()
+        for _ in range(args.plain_steps):
+            r.render()
+        pe = time.perf_counter() - t0
+        plain = frags_per_frame * args.plain_steps / pe / 1e9
+        print("[bench] plain %.3f ms/frame" % (pe * 1e3 / args.plain_steps), file=sys.stderr, flush=True)
+        ctx.set_mode(batched=True, tile=args.tile)
+
+    cpu = None
+    if not args.no_cpu:
+        from oracle import refrast
+        k = min(args.cpu_draws, len(scene.draws))
+        res = refrast.render(scene, threads=7, draws=scene.draws[:k], time_only=True)
+        cpu = {"value": round(res["counters"]["pixels_written"] / res["seconds"] / 1e9, 6), "unit": "Gfragments/s",
+               "cores": 8, "kind": "port",
+               "sample": "first %d of 256 draws of config3 (%d covered fragments, %.1f s), oracle threaded mode: "
+                         "7 workers + work-stealing submitter over 32x32 blocks" % (
+                             k, res["counters"]["pixels_written"], res["seconds"])}
+        print("[bench] cpu baseline %.4f Gfrag/s" % cpu["value"], file=sys.stderr, flush=True)
+
+    lat_ms = sorted(x * 1e3 for x in lat)
+    out = {
+        "metric": METRIC,
+        "value": round(value, 4),
+        "unit": "Gfragments/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic",
+        "config": {"workload": "config3: 1920x1080 BGRA8+D32, 256 draws, %d triangles, trilinear repeat, "
+                               "half alpha-blended" % scene.triangles,
+                   "mode": "batched", "tile": args.tile, "fragments_per_frame": frags_per_frame,
+                   "shaded_per_frame": stats_one["depth_passed"]},
+        "plain_per_draw_gfrag_s": round(plain, 4) if plain else None,
+        "speedup_vs_plain": round(value / plain, 2) if plain else None,
+        "frame_latency_ms": {"p50": round(statistics.median(lat_ms), 3),
+                             "p99": round(lat_ms[min(len(lat_ms) - 1, int(0.99 * len(lat_ms)))], 3)},
+        "kernel_ms": {"raster": round(raster_ms, 4), "setup": round(acc["setup_ms"] / max(args.steps, 1), 4),
+                      "bin": round(acc["bin_ms"] / max(args.steps, 1), 4)},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -53,6 +53,8 @@ typedef struct VhStats {
   uint64_t pixels_tested;   /* reference counter rasterizer/pixels/tested                  */
   uint64_t pixels_written;  /* covered fragments (rasterizer/pixels/written)               */
   uint64_t depth_passed;    /* shaded fragments (rasterizer/depth/passed)                  */
+  uint64_t shaded_blend;    /* shaded fragments of blending draws                          */
+  uint64_t shaded_dwrite;   /* shaded fragments that wrote depth                           */
 } VhStats;
 
 /* 1 when a HIP device is usable. */

@@ -43,3 +43,41 @@ def test_config1_bit_exact(ctx, batched):
     assert st["depth_passed"] == c["depth_passed"]
     assert st["tris_in"] == c["tris_in"] and st["tris_out"] == c["tris_out"]
     assert st["pixels_tested"] == c["pixels_tested"]
+
+
+def test_config2_small_bit_exact(ctx):
+    """Indexed sphere, MVP + normal matrix, Lambert x mipmapped texture, clamp, back-face cull."""
+    s = scenes.config2(stacks=40, slices=64, width=320, height=180, tex_size=64)
+    ref = refrast.render(s)
+    out = _render_gpu(ctx, s)
+    _compare(s, out, ref)
+    assert out["stats"]["pixels_written"] == ref["counters"]["pixels_written"] > 0
+    assert out["stats"]["tris_out"] == ref["counters"]["tris_out"] < ref["counters"]["tris_in"]
+
+
+@pytest.mark.parametrize("tile", [8, 16, 32])
+def test_config3_small_bit_exact(ctx, tile):
+    """Many draws, per-draw MVP, trilinear repeat, alternating opaque / alpha-blended."""
+    s = scenes.config3(ndraws=24, width=320, height=240, tex_size=64, max_tris=2048)
+    ref = refrast.render(s)
+    out = _render_gpu(ctx, s, tile=tile)
+    _compare(s, out, ref)
+    assert out["stats"]["depth_passed"] == ref["counters"]["depth_passed"]
+
+
+def test_config5_small_bit_exact(ctx):
+    """RGBA16F target, micro-triangles, 8 trilinear taps of an array texture, additive draws."""
+    s = scenes.config5(ntris=20000, ndraws=4, width=256, height=256, tex_size=64, layers=4)
+    ref = refrast.render(s)
+    out = _render_gpu(ctx, s)
+    _compare(s, out, ref)
+
+
+def test_config3_full_batched_equals_plain(ctx):
+    """At full size: batched and plain per-draw execution give identical bytes."""
+    s = scenes.config3()
+    a = _render_gpu(ctx, s, batched=True)
+    b = _render_gpu(ctx, s, batched=False)
+    assert np.array_equal(a["color"], b["color"])
+    assert np.array_equal(a["depth"].view(np.uint32), b["depth"].view(np.uint32))
+    assert a["stats"]["pixels_written"] == b["stats"]["pixels_written"]

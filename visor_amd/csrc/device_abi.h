@@ -90,6 +90,9 @@ typedef struct VaFrame {
 typedef struct VaCounters {
   unsigned long long draws, tris_in, tris_out, blocks;
   unsigned long long pixels_tested, pixels_written, depth_passed, pad;
+  unsigned long long shaded_blend;   /* shaded fragments of draws with blending on */
+  unsigned long long shaded_dwrite;  /* shaded fragments that wrote depth */
+  unsigned long long pad2[2];
 } VaCounters;
 
 #endif

@@ -717,6 +717,8 @@ VkResult Queue::RunPass(const std::vector<Op *> &passes, bool fused_clear, ExecS
   st.ctr.pixels_tested += c.pixels_tested;
   st.ctr.pixels_written += c.pixels_written;
   st.ctr.depth_passed += c.depth_passed;
+  st.ctr.shaded_blend += c.shaded_blend;
+  st.ctr.shaded_dwrite += c.shaded_dwrite;
   return VK_SUCCESS;
 }
 
@@ -789,6 +791,12 @@ VkResult Queue::Execute(std::vector<Submission> &subs)
     accum_stats.ctr.pixels_written += st.ctr.pixels_written;
     accum_stats.ctr.depth_passed += st.ctr.depth_passed;
     accum_stats.ctr.tris_in += st.ctr.tris_in;
+    accum_stats.ctr.tris_out += st.ctr.tris_out;
+    accum_stats.ctr.draws += st.ctr.draws;
+    accum_stats.ctr.blocks += st.ctr.blocks;
+    accum_stats.ctr.pixels_tested += st.ctr.pixels_tested;
+    accum_stats.ctr.shaded_blend += st.ctr.shaded_blend;
+    accum_stats.ctr.shaded_dwrite += st.ctr.shaded_dwrite;
     SignalFence(sub.fence, res);
   }
   return res;

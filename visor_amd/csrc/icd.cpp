@@ -1291,6 +1291,8 @@ int vh_get_stats(VkQueue queue, int which, VhStats *out)
   out->pixels_tested = s.ctr.pixels_tested;
   out->pixels_written = s.ctr.pixels_written;
   out->depth_passed = s.ctr.depth_passed;
+  out->shaded_blend = s.ctr.shaded_blend;
+  out->shaded_dwrite = s.ctr.shaded_dwrite;
   return 0;
 }
 

@@ -316,7 +316,7 @@ VA_INLINE void va_raster_tile(const VaDraw *__restrict__ draws, const VaFrame *_
       dep = (fused && (fr.clear_flags & VA_CLEAR_DEPTH)) ? fr.clear_depth : ((const float *)fr.depth)[pix];
   }
 
-  u32 n_cov = 0, n_pass = 0;
+  u32 n_cov = 0, n_pass = 0, n_blend = 0, n_dw = 0;
   for(u32 i = beg; i < en; ++i)
   {
     const u32 t = __builtin_amdgcn_readfirstlane(bins[i]);
@@ -358,6 +358,7 @@ VA_INLINE void va_raster_tile(const VaDraw *__restrict__ draws, const VaFrame *_
     const bool keep = va_fs(d.slot, d, (const float *)(r + VA_REC_HDR), b, o);
     if(!keep) continue;
     n_pass++;
+    n_blend += d.blend_enable ? 1u : 0u;
     if(hdr)
     {
       if(d.blend_enable)
@@ -392,7 +393,10 @@ VA_INLINE void va_raster_tile(const VaDraw *__restrict__ draws, const VaFrame *_
       c0 = (c0 & 0xff000000u) | (va_unorm_out(o[0]) << 16) | (va_unorm_out(o[1]) << 8) | va_unorm_out(o[2]);
     }
     if(d.depth_write && has_depth)
+    {
       dep = pixdepth;
+      n_dw++;
+    }
   }
 
   if(inside)
@@ -409,10 +413,13 @@ VA_INLINE void va_raster_tile(const VaDraw *__restrict__ draws, const VaFrame *_
       ((float *)fr.depth)[pix] = dep;
   }
   u64 s_cov = va_wave_sum((u64)n_cov), s_pass = va_wave_sum((u64)n_pass);
+  u64 s_blend = va_wave_sum((u64)n_blend), s_dw = va_wave_sum((u64)n_dw);
   if(lane == 0u && (s_cov | s_pass))
   {
     atomicAdd(&ctr->pixels_written, s_cov);
     atomicAdd(&ctr->depth_passed, s_pass);
+    atomicAdd(&ctr->shaded_blend, s_blend);
+    atomicAdd(&ctr->shaded_dwrite, s_dw);
   }
 }
 

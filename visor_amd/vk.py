@@ -228,7 +228,7 @@ class VhStats(C.Structure):
     _fields_ = [("gpu_ms", C.c_double), ("raster_ms", C.c_double), ("setup_ms", C.c_double), ("bin_ms", C.c_double),
                 ("raster_launches", u64), ("triangles", u64), ("bin_pairs", u64), ("tiles", u64), ("draws", u64),
                 ("tris_in", u64), ("tris_out", u64), ("blocks", u64), ("pixels_tested", u64), ("pixels_written", u64),
-                ("depth_passed", u64)]
+                ("depth_passed", u64), ("shaded_blend", u64), ("shaded_dwrite", u64)]
 
 
 P = C.POINTER
