@@ -48,7 +48,19 @@ def main():
     ap.add_argument("--cpu-draws", type=int, default=128, help="draws in the CPU-baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--calib-copy", action="store_true",
+                    help="run only a 1 GiB device copy (2 GiB of HBM traffic) to calibrate FETCH_SIZE/WRITE_SIZE")
     args = ap.parse_args()
+    if args.calib_copy:
+        import torch
+        a = torch.ones(1 << 28, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"calib_copy_bytes_read": a.numel() * 4, "calib_copy_bytes_written": b.numel() * 4,
+                          "launches": 3}))
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU: every box here has one MI355X; --gpus %d is not supported" % args.gpus,
               file=sys.stderr)
@@ -74,6 +86,7 @@ def main():
     for _ in range(args.warmup):
         r.render()
     stats_one = ctx.stats(0)
+    print("[bench] frame stats: %s" % json.dumps(stats_one), file=sys.stderr, flush=True)
 
     # timed batched steps
     ctx.icd.vh_reset_stats(ctx.queue)

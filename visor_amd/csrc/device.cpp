@@ -213,6 +213,10 @@ static std::string ModuleSource(const std::vector<std::shared_ptr<ShaderPair>> &
   s << "#define VA_MAXNV " << maxnv << "\n";
   s << "#define VA_REC_STRIDE " << rec_stride << "u\n";
   s << "#define VA_FS_DERIV " << (deriv ? 1 : 0) << "\n";
+  bool kill = false;
+  for(auto &p : pairs)
+    kill |= p->fs.has_kill;
+  s << "#define VA_FS_KILL " << (kill ? 1 : 0) << "\n";
   for(auto &p : pairs)
     s << p->vs.code << "\n" << p->fs.code << "\n";
   s << "VA_INLINE void va_vs(u32 slot, const VaDraw &d, u32 vtx, float pos[4], float *vary)\n{\n";

@@ -68,7 +68,8 @@ typedef struct VaDraw {
   unsigned int blend_enable, src_factor, dst_factor, blend_op;
   unsigned int frame;      /* index into the VaFrame table */
   unsigned int slot;       /* shader pair slot inside the pipeline module */
-  unsigned int pad[2];
+  unsigned int block_base; /* first setup workgroup of this draw (256 primitives each) */
+  unsigned int pad;
   unsigned char push[128];
   unsigned long long ubo[VA_MAX_UBO];
   unsigned long long ubo_bytes[VA_MAX_UBO];
@@ -88,11 +89,12 @@ typedef struct VaFrame {
 } VaFrame;
 
 typedef struct VaCounters {
-  unsigned long long draws, tris_in, tris_out, blocks;
-  unsigned long long pixels_tested, pixels_written, depth_passed, pad;
+  unsigned long long draws;
+  unsigned long long tris_in, tris_out, blocks, pixels_tested;  /* setup: per-workgroup partials */
+  unsigned long long pixels_written, depth_passed;                /* raster: per-tile partials     */
   unsigned long long shaded_blend;   /* shaded fragments of draws with blending on */
   unsigned long long shaded_dwrite;  /* shaded fragments that wrote depth */
-  unsigned long long pad2[2];
+  unsigned long long pad[3];
 } VaCounters;
 
 #endif

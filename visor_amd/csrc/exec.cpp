@@ -61,7 +61,8 @@ Queue::~Queue()
     qcv.notify_all();
     worker.join();
   }
-  void *bufs[] = {d_frames, d_draws, d_rec, d_count, d_offs, d_keys, d_vals, d_keys2, d_vals2, d_start, d_end, d_ctr, d_temp};
+  void *bufs[] = {d_frames, d_draws, d_rec, d_count, d_offs, d_keys, d_vals, d_keys2, d_vals2, d_start, d_end, d_ctr, d_temp,
+                  d_blocks, d_part};
   for(void *b : bufs)
     if(b)
       hipFree(b);
@@ -545,6 +546,7 @@ VkResult Queue::RunPass(const std::vector<Op *> &passes, bool fused_clear, ExecS
   std::vector<VaFrame> frames;
   std::vector<VaDraw> draws;
   std::vector<std::shared_ptr<ShaderPair>> pairs;
+  std::vector<uint32_t> block_draw;  // setup workgroup -> draw
   uint32_t tile_total = 0, tri_total = 0;
   for(Op *p : passes)
   {
@@ -584,6 +586,9 @@ VkResult Queue::RunPass(const std::vector<Op *> &passes, bool fused_clear, ExecS
       v.slot = (uint32_t)(it - pairs.begin());
       v.tri_base = tri_total;
       tri_total += v.ntris;
+      v.block_base = (uint32_t)block_draw.size();
+      for(uint32_t b = 0; b < (v.ntris + 255) / 256; b++)
+        block_draw.push_back((uint32_t)draws.size());
       draws.push_back(v);
     }
     frames.push_back(fr);
@@ -621,7 +626,9 @@ VkResult Queue::RunPass(const std::vector<Op *> &passes, bool fused_clear, ExecS
   if(!Grow(&d_frames, c_frames, sizeof(VaFrame) * nframes) || !Grow(&d_draws, c_draws, sizeof(VaDraw) * std::max(ndraws, 1u)) ||
      !Grow(&d_rec, c_rec, (size_t)stride * 4 * std::max(tri_total, 1u)) || !Grow(&d_count, c_count, 4ull * (tri_total + 1)) ||
      !Grow(&d_offs, c_offs, 4ull * (tri_total + 1)) || !Grow(&d_start, c_start, 4ull * tile_total) ||
-     !Grow(&d_end, c_end, 4ull * tile_total) || !Grow(&d_ctr, c_ctr, sizeof(VaCounters)))
+     !Grow(&d_end, c_end, 4ull * tile_total) || !Grow(&d_ctr, c_ctr, sizeof(VaCounters)) ||
+     !Grow(&d_blocks, c_blocks, 4ull * std::max<size_t>(block_draw.size(), 1)) ||
+     !Grow(&d_part, c_part, 32ull * (std::max<size_t>(block_draw.size(), tile_total) + 1)))
     return VK_ERROR_OUT_OF_DEVICE_MEMORY;
   if(!h_pinned)
     HIPCHK(hipHostMalloc((void **)&h_pinned, 4096, hipHostMallocDefault));
@@ -632,6 +639,8 @@ VkResult Queue::RunPass(const std::vector<Op *> &passes, bool fused_clear, ExecS
   HIPCHK(hipMemcpyAsync(d_frames, frames.data(), sizeof(VaFrame) * nframes, hipMemcpyHostToDevice, stream));
   if(ndraws)
     HIPCHK(hipMemcpyAsync(d_draws, draws.data(), sizeof(VaDraw) * ndraws, hipMemcpyHostToDevice, stream));
+  if(!block_draw.empty())
+    HIPCHK(hipMemcpyAsync(d_blocks, block_draw.data(), 4 * block_draw.size(), hipMemcpyHostToDevice, stream));
   HIPCHK(va_fill_u32_async(d_ctr, sizeof(VaCounters) / 4, 0u, stream));
   HIPCHK(hipEventRecord(ev[0], stream));
   uint32_t total_pairs = 0;
@@ -641,8 +650,11 @@ VkResult Queue::RunPass(const std::vector<Op *> &passes, bool fused_clear, ExecS
     VaFrame *df = (VaFrame *)d_frames;
     uint32_t *rec = (uint32_t *)d_rec, *cnt = (uint32_t *)d_count;
     VaCounters *ctr = (VaCounters *)d_ctr;
-    void *args[] = {&dd, (void *)&ndraws, &df, &tri_total, (void *)&ts, &rec, &cnt, &ctr};
-    HIPCHK(hipModuleLaunchKernel(mod->setup, (tri_total + 255) / 256, 1, 1, 256, 1, 1, 0, stream, args, nullptr));
+    uint32_t *bd = (uint32_t *)d_blocks;
+    uint64_t *part = (uint64_t *)d_part;
+    void *args[] = {&dd, &bd, &df, (void *)&ts, &rec, &cnt, &part};
+    HIPCHK(hipModuleLaunchKernel(mod->setup, (uint32_t)block_draw.size(), 1, 1, 256, 1, 1, 0, stream, args, nullptr));
+    HIPCHK(va_reduce_cols_async(part, (uint32_t)block_draw.size(), 4, (uint64_t *)&ctr->tris_in, stream));
     HIPCHK(va_fill_u32_async(cnt + tri_total, 1, 0u, stream));
     size_t need = 0;
     HIPCHK(va_scan_u32(nullptr, &need, cnt, (uint32_t *)d_offs, tri_total + 1, stream));
@@ -684,12 +696,14 @@ VkResult Queue::RunPass(const std::vector<Op *> &passes, bool fused_clear, ExecS
     VaFrame *df = (VaFrame *)d_frames;
     uint32_t *rec = (uint32_t *)d_rec, *bins = (uint32_t *)d_vals2, *st_ = (uint32_t *)d_start, *en = (uint32_t *)d_end;
     VaCounters *ctr = (VaCounters *)d_ctr;
+    uint64_t *part = (uint64_t *)d_part;
     uint32_t use_fused = fused_clear ? 1u : 0u;
-    void *args[] = {&dd, &df, (void *)&nframes, &rec, &bins, &st_, &en, &tile_total, &use_fused, &ctr};
+    void *args[] = {&dd, &df, (void *)&nframes, &rec, &bins, &st_, &en, &tile_total, &use_fused, &part};
     hipFunction_t k = mod->raster[ts - 3];
     uint32_t threads = (T / 8) * (T / 8) * 64;
     HIPCHK(hipModuleLaunchKernel(k, tile_total, 1, 1, threads, 1, 1, 0, stream, args, nullptr));
     HIPCHK(hipEventRecord(ev[3], stream));
+    HIPCHK(va_reduce_cols_async(part, tile_total, 4, (uint64_t *)&ctr->pixels_written, stream));
   }
   HIPCHK(hipMemcpyAsync(h_pinned + 64, d_ctr, sizeof(VaCounters), hipMemcpyDeviceToHost, stream));
   HIPCHK(hipStreamSynchronize(stream));

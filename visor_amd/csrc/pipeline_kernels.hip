@@ -49,6 +49,33 @@ VA_INLINE u64 va_wave_sum(u64 v)
   return v;
 }
 
+// Sums four counters over the workgroup and writes them as one row (every workgroup writes
+// its row, so the partials need no clearing).
+VA_INLINE void va_block_partials(u64 *row, u64 a, u64 b, u64 c, u64 e)
+{
+  __shared__ u64 s_part[16][4];
+  a = va_wave_sum(a);
+  b = va_wave_sum(b);
+  c = va_wave_sum(c);
+  e = va_wave_sum(e);
+  const u32 wave = threadIdx.x >> 6, nw = (blockDim.x + 63u) >> 6;
+  if((threadIdx.x & 63u) == 0u)
+  {
+    s_part[wave][0] = a;
+    s_part[wave][1] = b;
+    s_part[wave][2] = c;
+    s_part[wave][3] = e;
+  }
+  __syncthreads();
+  if(threadIdx.x < 4u)
+  {
+    u64 t = 0;
+    for(u32 w = 0; w < nw; w++)
+      t += s_part[w][threadIdx.x];
+    row[threadIdx.x] = t;
+  }
+}
+
 // reference rasterizer.cpp:100-119 (GetIndex, vertexOffset not applied)
 VA_INLINE u32 va_index(const VaDraw &d, u32 i)
 {
@@ -58,17 +85,20 @@ VA_INLINE u32 va_index(const VaDraw &d, u32 i)
   return 2ull * i + 2ull <= d.ib_bytes ? ((const u16 *)d.ib)[i] : 0u;
 }
 
+// One workgroup covers 256 primitives of a single draw (block_draw maps workgroup -> draw), so
+// the draw's state (push constants, vertex layout, descriptors) is read with scalar loads.
 extern "C" __global__ __launch_bounds__(256) void va_setup(
-    const VaDraw *__restrict__ draws, u32 ndraws, const VaFrame *__restrict__ frames, u32 total_tris,
-    u32 tile_shift, u32 *__restrict__ rec, u32 *__restrict__ tile_count, VaCounters *__restrict__ ctr)
+    const VaDraw *__restrict__ draws, const u32 *__restrict__ block_draw, const VaFrame *__restrict__ frames,
+    u32 tile_shift, u32 *__restrict__ rec, u32 *__restrict__ tile_count, u64 *__restrict__ part)
 {
-  const u32 g = blockIdx.x * blockDim.x + threadIdx.x;
+  const u32 di = __builtin_amdgcn_readfirstlane(block_draw[blockIdx.x]);
+  const VaDraw &d = draws[di];
+  const u32 prim = (blockIdx.x - d.block_base) * 256u + threadIdx.x;
+  const u32 g = d.tri_base + prim;
   u64 n_in = 0, n_out = 0, n_tested = 0, n_blocks = 0;
-  if(g < total_tris)
+  if(prim < d.ntris)
   {
-    const VaDraw &d = draws[va_find_draw(draws, ndraws, g)];
     const VaFrame &fr = frames[d.frame];
-    const u32 prim = g - d.tri_base;
     u32 p0, p1, p2;
     if(d.flags & VA_DRAW_STRIP)
     {
@@ -146,7 +176,7 @@ extern "C" __global__ __launch_bounds__(256) void va_setup(
           r[VA_R_IW0 + 2] = va_f2u(iw2);
           r[VA_R_BBMIN] = (u32)minx | ((u32)miny << 16);
           r[VA_R_BBMAX] = (u32)maxx | ((u32)maxy << 16);
-          r[VA_R_DRAW] = va_find_draw(draws, ndraws, g);
+          r[VA_R_DRAW] = di;
 #pragma unroll
           for(int j = 0; j < VA_MAXNV; j++)
           {
@@ -162,17 +192,9 @@ extern "C" __global__ __launch_bounds__(256) void va_setup(
     }
     tile_count[g] = cnt;
   }
-  n_in = va_wave_sum(n_in);
-  n_out = va_wave_sum(n_out);
-  n_tested = va_wave_sum(n_tested);
-  n_blocks = va_wave_sum(n_blocks);
-  if((threadIdx.x & 63u) == 0u && n_in)
-  {
-    atomicAdd(&ctr->tris_in, n_in);
-    atomicAdd(&ctr->tris_out, n_out);
-    atomicAdd(&ctr->pixels_tested, n_tested);
-    atomicAdd(&ctr->blocks, n_blocks);
-  }
+  // counters: one row of partial sums per workgroup (no contended atomics), columns in
+  // VaCounters order from tris_in
+  va_block_partials(part + (u64)blockIdx.x * 4u, n_in, n_out, n_blocks, n_tested);
 }
 
 // One thread per (tile, primitive) pair. offsets = exclusive scan of tile counts (ntris + 1).
@@ -219,7 +241,7 @@ extern "C" __global__ __launch_bounds__(256) void va_fill_u32(u32 *__restrict__ 
   if(i < n) p[i] = v;
 }
 
-VA_INLINE float va_unorm_in(u32 byte) { return (float)byte / 255.0f; }
+VA_INLINE float va_unorm_in(u32 byte) { return va_unorm8(byte); }
 
 // reference rasterizer.cpp:277-280,674-676 (clamp01, truncate)
 VA_INLINE u32 va_unorm_out(float v)
@@ -264,13 +286,124 @@ VA_INLINE bool va_bary(const u32 *__restrict__ r, i32 x, i32 y, float b[4], floa
   return covered;
 }
 
+// Blend (rasterizer.cpp:593-672) and store (:674-676) one shaded fragment into the pixel's
+// registers. 8-bit targets: dst = (byte[2], byte[1], byte[0]) / 255 with dst alpha 1, factors
+// from the colour factors, bytes B,G,R truncated, alpha byte untouched. RGBA16F targets (beyond
+// the reference) blend all four channels and store halves.
+VA_INLINE void va_blend_store(const VaDraw &d, bool hdr, float o[4], u32 &c0, u32 &c1)
+{
+  if(hdr)
+  {
+    if(d.blend_enable)
+    {
+      const float sf = va_blend_factor(d.src_factor, o[3]);
+      const float df_ = va_blend_factor(d.dst_factor, o[3]);
+      const float e0 = va_half_val(c0 & 0xffffu), e1 = va_half_val(c0 >> 16);
+      const float e2 = va_half_val(c1 & 0xffffu), e3 = va_half_val(c1 >> 16);
+      o[0] = sf * o[0] + df_ * e0;
+      o[1] = sf * o[1] + df_ * e1;
+      o[2] = sf * o[2] + df_ * e2;
+      o[3] = sf * o[3] + df_ * e3;
+    }
+    c0 = va_half_bits(o[0]) | (va_half_bits(o[1]) << 16);
+    c1 = va_half_bits(o[2]) | (va_half_bits(o[3]) << 16);
+  }
+  else
+  {
+    if(d.blend_enable)
+    {
+      const float sf = va_blend_factor(d.src_factor, o[3]);
+      const float df_ = va_blend_factor(d.dst_factor, o[3]);
+      const float e0 = va_unorm_in((c0 >> 16) & 0xffu);
+      const float e1 = va_unorm_in((c0 >> 8) & 0xffu);
+      const float e2 = va_unorm_in(c0 & 0xffu);
+      const float r0 = sf * o[0] + df_ * e0;
+      const float r1 = sf * o[1] + df_ * e1;
+      const float r2 = sf * o[2] + df_ * e2;
+      o[0] = r0;
+      o[1] = r1;
+      o[2] = r2;
+    }
+    c0 = (c0 & 0xff000000u) | (va_unorm_out(o[0]) << 16) | (va_unorm_out(o[1]) << 8) | va_unorm_out(o[2]);
+  }
+}
+
+// depth compare (rasterizer.cpp:562-576)
+VA_INLINE bool va_depth_pass(u32 op, float z, float cur)
+{
+  switch(op)
+  {
+    case 0u: return false;
+    case 1u: return z < cur;
+    case 2u: return z == cur;
+    case 3u: return z <= cur;
+    case 4u: return z > cur;
+    case 5u: return z != cur;
+    case 6u: return z >= cur;
+    default: return true;
+  }
+}
+
+VA_INLINE u32 va_lanes_below(u64 m)
+{
+  return __builtin_amdgcn_mbcnt_hi((u32)(m >> 32), __builtin_amdgcn_mbcnt_lo((u32)m, 0u));
+}
+
+// Orders this wave's LDS writes before its later LDS reads by other lanes.
+VA_INLINE void va_wave_sync()
+{
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+#ifndef VA_CHUNK
+#define VA_CHUNK 64u       // bin entries staged in LDS per round
+#endif
+#define VA_MAXJOBS 128u    // pending fragments per wave before a shading flush
+#define VA_SMALL 16        // bbox-in-block overlaps up to this many pixels are tested per triangle
+
+// One workgroup per tile; wave w owns the 8x8 block w of the tile, one pixel per lane, colour
+// and depth in registers. The workgroup stages VA_CHUNK bin entries' triangle records in LDS;
+// each wave then walks them in bin (API primitive) order: bbox reject, integer edge functions,
+// early depth test and depth write in order per pixel. Fragments that pass are appended to a
+// wave-local job list; a flush shades up to 64 of them per step regardless of which triangle or
+// pixel they belong to, then every pixel blends its own results in primitive order. The
+// arithmetic and order of every fragment are those of the reference; only the lane that runs
+// a fragment program changes.
+// Coverage and interpolated depth only (the first half of va_bary, same operations).
+VA_INLINE bool va_cov_depth(const u32 *__restrict__ r, i32 x, i32 y, float *depth)
+{
+  const i32 PAx = (i32)r[VA_R_X0] - x;
+  const i32 PAy = (i32)r[VA_R_Y0] - y;
+  const u32 ux = r[VA_R_ACX] * (u32)PAy - r[VA_R_ACY] * (u32)PAx;
+  const u32 uy = (u32)PAx * r[VA_R_ABY] - (u32)PAy * r[VA_R_ABX];
+  const i32 b0 = (i32)(r[VA_R_AREA] - (ux + uy)), b1 = (i32)ux, b2 = (i32)uy;
+  const float invarea = va_u2f(r[VA_R_INVAREA]);
+  float n0 = (float)b0, n1 = (float)b1, n2 = (float)b2;
+  n0 *= invarea;
+  n1 *= invarea;
+  n2 *= invarea;
+  *depth = n0 * va_u2f(r[VA_R_Z0]) + n1 * va_u2f(r[VA_R_Z0 + 1]) + n2 * va_u2f(r[VA_R_Z0 + 2]);
+  return b0 >= 0 && b1 >= 0 && b2 >= 0;
+}
+
 template <int T>
 VA_INLINE void va_raster_tile(const VaDraw *__restrict__ draws, const VaFrame *__restrict__ frames,
                               u32 nframes, const u32 *__restrict__ rec, const u32 *__restrict__ bins,
                               const u32 *__restrict__ bin_start, const u32 *__restrict__ bin_end,
-                              u32 use_fused_clear, VaCounters *__restrict__ ctr, u32 tile)
+                              u32 use_fused_clear, u64 *__restrict__ part, u32 tile)
 {
-  constexpr int SB = T / 8;
+  constexpr u32 SB = T / 8, NW = SB * SB, NT = NW * 64u;
+  constexpr u32 Q = VA_REC_STRIDE / 4u;
+  __shared__ uint4 s_rec4[VA_CHUNK * Q];
+  __shared__ u32 s_tri[VA_CHUNK];
+  __shared__ u64 s_mask[NW][VA_CHUNK];
+  __shared__ u16 s_base[NW][VA_CHUNK];
+  __shared__ u16 s_job[NW][VA_MAXJOBS];
+  __shared__ float4 s_res[NW][VA_MAXJOBS];
+  const u32 *s_rec = (const u32 *)s_rec4;
+
   const VaFrame &fr = frames[va_find_frame(frames, nframes, tile)];
   const u32 lt = tile - fr.tile_base;
   const u32 tx = lt % fr.tiles_x, ty = lt / fr.tiles_x;
@@ -281,7 +414,13 @@ VA_INLINE void va_raster_tile(const VaDraw *__restrict__ draws, const VaFrame *_
   const bool inside = (u32)px < W && (u32)py < H;
   const u32 beg = bin_start[tile], en = bin_end[tile];
   const bool fused = use_fused_clear != 0u && fr.clear_flags != 0u;
-  if(beg == en && !fused) return;
+  u64 *row = part + (u64)tile * 4u;
+  if(beg == en && !fused)
+  {
+    if(threadIdx.x < 4u)
+      row[threadIdx.x] = 0ull;
+    return;
+  }
 
   const u64 pix = (u64)(u32)py * W + (u32)px;
   const bool has_depth = fr.depth != 0ull;
@@ -317,86 +456,189 @@ VA_INLINE void va_raster_tile(const VaDraw *__restrict__ draws, const VaFrame *_
   }
 
   u32 n_cov = 0, n_pass = 0, n_blend = 0, n_dw = 0;
-  for(u32 i = beg; i < en; ++i)
+  for(u32 cb = beg; cb < en; cb += VA_CHUNK)
   {
-    const u32 t = __builtin_amdgcn_readfirstlane(bins[i]);
-    const u32 *__restrict__ r = rec + (u64)t * VA_REC_STRIDE;
-    const u32 bmin = r[VA_R_BBMIN], bmax = r[VA_R_BBMAX];
-    const i32 minx = (i32)(bmin & 0xffffu), miny = (i32)(bmin >> 16);
-    const i32 maxx = (i32)(bmax & 0xffffu), maxy = (i32)(bmax >> 16);
-    if(maxx <= bx || minx >= bx + 8 || maxy <= by || miny >= by + 8) continue;
-    float b[3][4];
-    float pixdepth;
-    bool cov = va_bary(r, px, py, b[0], &pixdepth);
-    cov = cov && inside && px >= minx && px < maxx && py >= miny && py < maxy;
-    if(__ballot(cov) == 0ull) continue;
-    if(!cov) continue;
-    const VaDraw &d = draws[r[VA_R_DRAW]];
-    n_cov++;
-    bool passed = true;
-    const u32 op = d.depth_op;
-    if(op != 7u && has_depth)
+    const u32 n = min(VA_CHUNK, en - cb);
+    __syncthreads();  // the previous chunk is consumed by every wave
+    for(u32 i = threadIdx.x; i < n; i += NT)
+      s_tri[i] = bins[cb + i];
+    __syncthreads();
+    for(u32 w = threadIdx.x; w < n * Q; w += NT)
     {
-      switch(op)
+      const u32 k = w / Q, q = w - k * Q;
+      s_rec4[w] = ((const uint4 *)(rec + (u64)s_tri[k] * VA_REC_STRIDE))[q];
+    }
+    __syncthreads();
+
+#if VA_FS_KILL
+    // fragment programs that may discard decide depth writes: shade in place, in order
+    for(u32 k = 0; k < n; ++k)
+    {
+      const u32 *r = s_rec + k * VA_REC_STRIDE;
+      const u32 bmin = r[VA_R_BBMIN], bmax = r[VA_R_BBMAX];
+      const i32 minx = (i32)(bmin & 0xffffu), miny = (i32)(bmin >> 16);
+      const i32 maxx = (i32)(bmax & 0xffffu), maxy = (i32)(bmax >> 16);
+      if(maxx <= bx || minx >= bx + 8 || maxy <= by || miny >= by + 8) continue;
+      float b[3][4];
+      float pixdepth;
+      bool cov = va_bary(r, px, py, b[0], &pixdepth);
+      cov = cov && inside && px >= minx && px < maxx && py >= miny && py < maxy;
+      if(__ballot(cov) == 0ull || !cov) continue;
+      const VaDraw &d = draws[r[VA_R_DRAW]];
+      n_cov++;
+      if(d.depth_op != 7u && has_depth && !va_depth_pass(d.depth_op, pixdepth, dep)) continue;
+      va_bary(r, px + 1, py, b[1], nullptr);
+      va_bary(r, px, py + 1, b[2], nullptr);
+      float o[4];
+      if(!va_fs(d.slot, d, (const float *)(r + VA_REC_HDR), b, o)) continue;
+      n_pass++;
+      n_blend += d.blend_enable ? 1u : 0u;
+      va_blend_store(d, hdr, o, c0, c1);
+      if(d.depth_write && has_depth)
       {
-        case 0u: passed = false; break;
-        case 1u: passed = pixdepth < dep; break;
-        case 2u: passed = pixdepth == dep; break;
-        case 3u: passed = pixdepth <= dep; break;
-        case 4u: passed = pixdepth > dep; break;
-        case 5u: passed = pixdepth != dep; break;
-        case 6u: passed = pixdepth >= dep; break;
-        default: break;
+        dep = pixdepth;
+        n_dw++;
       }
     }
-    if(!passed) continue;
+#else
+    u32 njobs = 0, k0 = 0;
+    auto flush = [&](u32 ka, u32 kb) {
+      va_wave_sync();
+      for(u32 j = lane; j < njobs; j += 64u)
+      {
+        const u32 job = s_job[wave][j];
+        const u32 k = job >> 6, pl = job & 63u;
+        const u32 *r = s_rec + k * VA_REC_STRIDE;
+        const i32 x = bx + (i32)(pl & 7u), y = by + (i32)(pl >> 3);
+        float b[3][4];
+        va_bary(r, x, y, b[0], nullptr);
 #if VA_FS_DERIV
-    va_bary(r, px + 1, py, b[1], nullptr);
-    va_bary(r, px, py + 1, b[2], nullptr);
+        va_bary(r, x + 1, y, b[1], nullptr);
+        va_bary(r, x, y + 1, b[2], nullptr);
 #endif
-    float o[4];
-    const bool keep = va_fs(d.slot, d, (const float *)(r + VA_REC_HDR), b, o);
-    if(!keep) continue;
-    n_pass++;
-    n_blend += d.blend_enable ? 1u : 0u;
-    if(hdr)
-    {
-      if(d.blend_enable)
-      {
-        const float sf = va_blend_factor(d.src_factor, o[3]);
-        const float df_ = va_blend_factor(d.dst_factor, o[3]);
-        const float e0 = va_half_val(c0 & 0xffffu), e1 = va_half_val(c0 >> 16);
-        const float e2 = va_half_val(c1 & 0xffffu), e3 = va_half_val(c1 >> 16);
-        o[0] = sf * o[0] + df_ * e0;
-        o[1] = sf * o[1] + df_ * e1;
-        o[2] = sf * o[2] + df_ * e2;
-        o[3] = sf * o[3] + df_ * e3;
+        const VaDraw &d = draws[r[VA_R_DRAW]];
+        float o[4];
+        va_fs(d.slot, d, (const float *)(r + VA_REC_HDR), b, o);
+        s_res[wave][j] = make_float4(o[0], o[1], o[2], o[3]);
       }
-      c0 = va_half_bits(o[0]) | (va_half_bits(o[1]) << 16);
-      c1 = va_half_bits(o[2]) | (va_half_bits(o[3]) << 16);
-    }
-    else
-    {
-      if(d.blend_enable)
+      va_wave_sync();
+      for(u32 k = ka; k < kb; ++k)
       {
-        // rasterizer.cpp:593-672: dst = (byte[2], byte[1], byte[0]) / 255, dst alpha 1
-        const float sf = va_blend_factor(d.src_factor, o[3]);
-        const float df_ = va_blend_factor(d.dst_factor, o[3]);
-        const float e0 = va_unorm_in((c0 >> 16) & 0xffu);
-        const float e1 = va_unorm_in((c0 >> 8) & 0xffu);
-        const float e2 = va_unorm_in(c0 & 0xffu);
-        const float r0 = sf * o[0] + df_ * e0;
-        const float r1 = sf * o[1] + df_ * e1;
-        const float r2 = sf * o[2] + df_ * e2;
-        o[0] = r0; o[1] = r1; o[2] = r2;
+        const u64 m = s_mask[wave][k];
+        if(!m) continue;
+        if((m >> lane) & 1ull)
+        {
+          const float4 v = s_res[wave][s_base[wave][k] + va_lanes_below(m)];
+          float o[4] = {v.x, v.y, v.z, v.w};
+          const VaDraw &d = draws[s_rec[k * VA_REC_STRIDE + VA_R_DRAW]];
+          va_blend_store(d, hdr, o, c0, c1);
+        }
       }
-      c0 = (c0 & 0xff000000u) | (va_unorm_out(o[0]) << 16) | (va_unorm_out(o[1]) << 8) | va_unorm_out(o[2]);
-    }
-    if(d.depth_write && has_depth)
+      va_wave_sync();
+    };
+    // Triangle-parallel pre-pass: lane k tests triangle k's bbox against this wave's block and,
+    // when the overlap is at most VA_SMALL pixels, evaluates the edge functions there itself
+    // (same integer arithmetic as the pixel-parallel test), giving a 64-bit pixel mask. Larger
+    // overlaps are marked "big" and tested pixel-parallel in the ordered pass below.
+    u32 cm_lo = 0u, cm_hi = 0u;
+    bool big = false;
+    if(lane < n)
     {
-      dep = pixdepth;
-      n_dw++;
+      const u32 *r = s_rec + lane * VA_REC_STRIDE;
+      const u32 bmin = r[VA_R_BBMIN], bmax = r[VA_R_BBMAX];
+      const i32 x0 = max((i32)(bmin & 0xffffu), bx), y0 = max((i32)(bmin >> 16), by);
+      const i32 x1 = min(min((i32)(bmax & 0xffffu), bx + 8), (i32)W);
+      const i32 y1 = min(min((i32)(bmax >> 16), by + 8), (i32)H);
+      if(x0 < x1 && y0 < y1)
+      {
+        if((x1 - x0) * (y1 - y0) > VA_SMALL)
+          big = true;
+        else
+        {
+          const i32 X0 = (i32)r[VA_R_X0], Y0 = (i32)r[VA_R_Y0];
+          const u32 ABX = r[VA_R_ABX], ABY = r[VA_R_ABY], ACX = r[VA_R_ACX], ACY = r[VA_R_ACY], AR = r[VA_R_AREA];
+          for(i32 y = y0; y < y1; y++)
+            for(i32 x = x0; x < x1; x++)
+            {
+              const i32 PAx = X0 - x, PAy = Y0 - y;
+              const u32 ux = ACX * (u32)PAy - ACY * (u32)PAx;
+              const u32 uy = (u32)PAx * ABY - (u32)PAy * ABX;
+              if((i32)(AR - (ux + uy)) >= 0 && (i32)ux >= 0 && (i32)uy >= 0)
+              {
+                const u32 bit = (u32)((y - by) * 8 + (x - bx));
+                if(bit < 32u) cm_lo |= 1u << bit; else cm_hi |= 1u << (bit - 32u);
+              }
+            }
+        }
+      }
+      s_mask[wave][lane] = 0ull;
     }
+    const u64 bigmask = __ballot(big);
+    u64 todo = __ballot((cm_lo | cm_hi) != 0u) | bigmask;
+    va_wave_sync();
+    // Ordered pass (primitive order): early depth test and depth write per pixel, job append.
+    while(todo)
+    {
+      const u32 k = (u32)__builtin_ctzll(todo);
+      todo &= todo - 1ull;
+      const u32 *r = s_rec + k * VA_REC_STRIDE;
+      float pixdepth;
+      bool cov;
+      if((bigmask >> k) & 1ull)
+      {
+        const u32 bmin = r[VA_R_BBMIN], bmax = r[VA_R_BBMAX];
+        cov = va_cov_depth(r, px, py, &pixdepth);
+        cov = cov && inside && px >= (i32)(bmin & 0xffffu) && px < (i32)(bmax & 0xffffu) && py >= (i32)(bmin >> 16) &&
+              py < (i32)(bmax >> 16);
+      }
+      else
+      {
+        const u32 lo = (u32)__builtin_amdgcn_readlane((int)cm_lo, (int)k);
+        const u32 hi = (u32)__builtin_amdgcn_readlane((int)cm_hi, (int)k);
+        cov = ((lane < 32u ? lo >> lane : hi >> (lane - 32u)) & 1u) != 0u;
+        if(cov)
+          va_cov_depth(r, px, py, &pixdepth);
+      }
+      if(__ballot(cov) == 0ull)
+        continue;
+      const VaDraw &d = draws[r[VA_R_DRAW]];
+      bool passed = cov;
+      if(cov)
+      {
+        n_cov++;
+        if(d.depth_op != 7u && has_depth)
+          passed = va_depth_pass(d.depth_op, pixdepth, dep);
+      }
+      const u64 m = __ballot(passed);
+      if(!m)
+        continue;
+      const u32 cnt = (u32)__popcll(m);
+      if(njobs + cnt > VA_MAXJOBS)
+      {
+        flush(k0, k);
+        njobs = 0;
+        k0 = k;
+      }
+      if(passed)
+      {
+        s_job[wave][njobs + va_lanes_below(m)] = (u16)((k << 6) | lane);
+        n_pass++;
+        n_blend += d.blend_enable ? 1u : 0u;
+        if(d.depth_write && has_depth)
+        {
+          dep = pixdepth;
+          n_dw++;
+        }
+      }
+      if(lane == 0u)
+      {
+        s_base[wave][k] = (u16)njobs;
+        s_mask[wave][k] = m;
+      }
+      njobs += cnt;
+    }
+    flush(k0, n);
+#endif
   }
 
   if(inside)
@@ -412,26 +654,19 @@ VA_INLINE void va_raster_tile(const VaDraw *__restrict__ draws, const VaFrame *_
     if(has_depth)
       ((float *)fr.depth)[pix] = dep;
   }
-  u64 s_cov = va_wave_sum((u64)n_cov), s_pass = va_wave_sum((u64)n_pass);
-  u64 s_blend = va_wave_sum((u64)n_blend), s_dw = va_wave_sum((u64)n_dw);
-  if(lane == 0u && (s_cov | s_pass))
-  {
-    atomicAdd(&ctr->pixels_written, s_cov);
-    atomicAdd(&ctr->depth_passed, s_pass);
-    atomicAdd(&ctr->shaded_blend, s_blend);
-    atomicAdd(&ctr->shaded_dwrite, s_dw);
-  }
+  // columns in VaCounters order from pixels_written
+  va_block_partials(row, n_cov, n_pass, n_blend, n_dw);
 }
 
 #define VA_RASTER_KERNEL(T)                                                                                 \
   extern "C" __global__ __launch_bounds__((T / 8) * (T / 8) * 64) void va_raster_##T(                      \
       const VaDraw *__restrict__ draws, const VaFrame *__restrict__ frames, u32 nframes,                  \
       const u32 *__restrict__ rec, const u32 *__restrict__ bins, const u32 *__restrict__ bin_start,        \
-      const u32 *__restrict__ bin_end, u32 ntiles, u32 use_fused_clear, VaCounters *__restrict__ ctr)      \
+      const u32 *__restrict__ bin_end, u32 ntiles, u32 use_fused_clear, u64 *__restrict__ part)            \
   {                                                                                                         \
     const u32 tile = blockIdx.x;                                                                            \
     if(tile < ntiles)                                                                                       \
-      va_raster_tile<T>(draws, frames, nframes, rec, bins, bin_start, bin_end, use_fused_clear, ctr, tile); \
+      va_raster_tile<T>(draws, frames, nframes, rec, bins, bin_start, bin_end, use_fused_clear, part, tile); \
   }
 
 VA_RASTER_KERNEL(8)

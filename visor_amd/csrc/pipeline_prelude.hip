@@ -17,6 +17,19 @@ VA_INLINE float va_u2f(u32 u) { return __uint_as_float(u); }
 VA_INLINE u32 va_f2u(float f) { return __float_as_uint(f); }
 VA_INLINE i32 va_f2i(float f) { return (i32)f; }
 
+// byte / 255.0f, correctly rounded, without a division: q = b * RN(1/255), then one Newton
+// correction with exact FMAs. Equal to the IEEE quotient for every b in 0..255 (checked
+// exhaustively by the test suite), so texel and attribute values match the reference's
+// float(byte) / 255.0f bit for bit.
+VA_INLINE float va_unorm8(u32 b)
+{
+  const float c = 0.0039215688593685627f;  // RN(1/255)
+  const float x = (float)b;
+  const float q = x * c;
+  const float r = __builtin_fmaf(-q, 255.0f, x);
+  return __builtin_fmaf(r, c, q);
+}
+
 // ---------------------------------------------------------------------------------------------
 // Triple-point floats: value at the pixel and at its right and lower neighbours. Fragment
 // programs that take implicit-LOD samples or derivatives run on these; the differences give
@@ -101,10 +114,10 @@ VA_INLINE void va_vattr(const VaDraw &d, u32 vtx, u32 loc, float out[4])
     case 37:
     {
       u32 u = w[0];
-      out[0] = (float)((u & 0x000000ffu) >> 0) / 255.0f;
-      out[1] = (float)((u & 0x0000ff00u) >> 8) / 255.0f;
-      out[2] = (float)((u & 0x00ff0000u) >> 16) / 255.0f;
-      out[3] = (float)((u & 0xff000000u) >> 24) / 255.0f;
+      out[0] = va_unorm8((u & 0x000000ffu) >> 0);
+      out[1] = va_unorm8((u & 0x0000ff00u) >> 8);
+      out[2] = va_unorm8((u & 0x00ff0000u) >> 16);
+      out[3] = va_unorm8((u & 0xff000000u) >> 24);
       break;
     }
     default: break;
@@ -119,10 +132,10 @@ VA_INLINE void va_vattr(const VaDraw &d, u32 vtx, u32 loc, float out[4])
 VA_INLINE void va_texel(const u8 *base, u32 W, i32 x, i32 y, float out[4])
 {
   u32 u = *(const u32 *)(base + ((u64)(u32)y * W + (u32)x) * 4u);
-  out[0] = (float)(u & 0xffu) / 255.0f;
-  out[1] = (float)((u >> 8) & 0xffu) / 255.0f;
-  out[2] = (float)((u >> 16) & 0xffu) / 255.0f;
-  out[3] = (float)(u >> 24) / 255.0f;
+  out[0] = va_unorm8(u & 0xffu);
+  out[1] = va_unorm8((u >> 8) & 0xffu);
+  out[2] = va_unorm8((u >> 16) & 0xffu);
+  out[3] = va_unorm8(u >> 24);
 }
 
 VA_INLINE void va_sample_level(const VaTex &t, u32 level, u32 layer, float u, float v, u32 filter,

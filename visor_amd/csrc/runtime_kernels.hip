@@ -63,3 +63,45 @@ extern "C" hipError_t va_fill_u64_async(void *p, uint64_t count, uint32_t lo, ui
                      make_uint2(lo, hi));
   return hipGetLastError();
 }
+
+// out[c] += sum over rows of part[row * 4 + c] (four columns). Each workgroup reduces a slice
+// of rows and adds its four totals atomically: a few dozen atomics instead of one per wave.
+__global__ __launch_bounds__(256) void va_reduce4_kernel(const uint64_t *part, uint32_t rows, uint64_t *out)
+{
+  __shared__ uint64_t s[4][256];
+  uint64_t t[4] = {0, 0, 0, 0};
+  for(uint32_t r = blockIdx.x * blockDim.x + threadIdx.x; r < rows; r += gridDim.x * blockDim.x)
+  {
+    const ulonglong2 *p = (const ulonglong2 *)(part + (uint64_t)r * 4u);
+    ulonglong2 a = p[0], b = p[1];
+    t[0] += a.x;
+    t[1] += a.y;
+    t[2] += b.x;
+    t[3] += b.y;
+  }
+  for(int c = 0; c < 4; c++)
+    s[c][threadIdx.x] = t[c];
+  __syncthreads();
+  for(uint32_t o = blockDim.x / 2; o > 0; o >>= 1)
+  {
+    if(threadIdx.x < o)
+      for(int c = 0; c < 4; c++)
+        s[c][threadIdx.x] += s[c][threadIdx.x + o];
+    __syncthreads();
+  }
+  if(threadIdx.x < 4)
+    atomicAdd((unsigned long long *)&out[threadIdx.x], (unsigned long long)s[threadIdx.x][0]);
+}
+
+// Adds the column sums of a rows x 4 partials array into out[0..3] (out must start at zero).
+extern "C" hipError_t va_reduce_cols_async(const uint64_t *part, uint32_t rows, uint32_t cols, uint64_t *out,
+                                           hipStream_t stream)
+{
+  if(cols != 4)
+    return hipErrorInvalidValue;
+  uint32_t blocks = (rows + 1023) / 1024;
+  if(blocks < 1) blocks = 1;
+  if(blocks > 64) blocks = 64;
+  hipLaunchKernelGGL(va_reduce4_kernel, dim3(blocks), dim3(256), 0, stream, part, rows, out);
+  return hipGetLastError();
+}
