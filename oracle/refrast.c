@@ -131,13 +131,17 @@ float rr_log2(float x)
 typedef struct tc_entry {
   struct tc_entry *prev, *next;
   int tlx, tly;
-  uint64_t byte_offs;
-  const rr_image *tex;
+  const uint8_t *base;
   float px[4][4][4];
 } tc_entry;
 
 static _Thread_local tc_entry tc_data[8];
 static _Thread_local tc_entry *tc_head;
+/* Every public entry point bumps the generation, so no thread can hit on a block cached from
+ * image memory that the caller has since freed or rewritten (the key is the subresource's base
+ * address, which a later allocation can reuse). */
+static _Atomic uint64_t tc_generation = 1;
+static _Thread_local uint64_t tc_seen;
 
 static void tc_init(void)
 {
@@ -154,13 +158,18 @@ static void tc_init(void)
 static const float *tc_fetch(const rr_image *tex, uint64_t byte_offs, uint32_t row_w,
                             uint32_t row_h, int x, int y)
 {
-  if(!tc_head)
+  uint64_t gen = tc_generation;
+  if(!tc_head || tc_seen != gen)
+  {
     tc_init();
+    tc_seen = gen;
+  }
   int tlx = x & ~3, tly = y & ~3;
+  const uint8_t *base = tex->pixels + byte_offs;
   tc_entry *e = tc_head;
   for(;;)
   {
-    if(e->tex == tex && e->byte_offs == byte_offs && e->tlx == tlx && e->tly == tly)
+    if(e->base == base && e->tlx == tlx && e->tly == tly)
     {
       if(e->prev)
       {
@@ -183,13 +192,11 @@ static const float *tc_fetch(const rr_image *tex, uint64_t byte_offs, uint32_t r
   e->prev = NULL;
   e->next = tc_head;
   tc_head = e;
-  e->tex = tex;
-  e->byte_offs = byte_offs;
+  e->base = base;
   e->tlx = tlx;
   e->tly = tly;
   /* The reference fills the whole 4x4 block, reading past the row or image end at the edges;
    * here only in-bounds texels are read (the others are never returned for in-bounds x, y). */
-  const uint8_t *base = tex->pixels + byte_offs;
   for(int r = 0; r < 4; r++)
     for(int c = 0; c < 4; c++)
     {
@@ -224,6 +231,7 @@ static void texel(const rr_image *tex, uint64_t byte_offs, uint32_t row_w, uint3
  * too (the reference reads one texel past the row there). */
 void rr_sample_tex_wrapped(float u, float v, const rr_image *tex, uint64_t byte_offs, float out[4])
 {
+  atomic_fetch_add(&tc_generation, 1);
   const uint32_t W = tex->width, H = tex->height;
   u = u - floorf(u);
   v = v - floorf(v);
@@ -328,8 +336,8 @@ static void sample_level(const rr_image *img, uint32_t level, uint32_t layer, fl
 /* Full sampler (beyond the reference, which samples mip 0 of layer 0 only): LOD from the
  * coordinate's screen-space differences, Vulkan 1.0 level selection and filtering (15.6.7-8),
  * array layer = clamp(floor(z + 0.5)). */
-void rr_sample(const rr_image *img, const rr_sampler *s, const float coord[3], const float ddx[2],
-               const float ddy[2], int use_lod, float out[4])
+static void sample_full(const rr_image *img, const rr_sampler *s, const float coord[3],
+                        const float ddx[2], const float ddy[2], int use_lod, float out[4])
 {
   float lambda = 0.0f;
   if(use_lod)
@@ -533,7 +541,7 @@ static void tex_uv(const rr_state *st, int binding, const float b3[3][4], const 
   c[0] = u0; c[1] = v0; c[2] = 0.0f;
   ddx[0] = u1 - u0; ddx[1] = v1 - v0;
   ddy[0] = u2 - u0; ddy[1] = v2 - v0;
-  rr_sample(st->tex[binding], st->samp[binding], c, ddx, ddy, 1, out);
+  sample_full(st->tex[binding], st->samp[binding], c, ddx, ddy, 1, out);
 }
 
 static void fs_program(const rr_state *st, float pixdepth, const float b3[3][4],
@@ -586,7 +594,7 @@ static void fs_program(const rr_state *st, float pixdepth, const float b3[3][4],
         float ddx[2] = {cx[1] - cx[0], cy[1] - cy[0]};
         float ddy[2] = {cx[2] - cx[0], cy[2] - cy[0]};
         float t[4];
-        rr_sample(st->tex[0], st->samp[0], c, ddx, ddy, 1, t);
+        sample_full(st->tex[0], st->samp[0], c, ddx, ddy, 1, t);
         for(int i = 0; i < 4; i++)
           acc[i] = acc[i] + t[i] * kTapWeight[k];
       }
@@ -995,6 +1003,7 @@ static void emit_serial(void *ctx, const tri_work *w) { fifo_push((fifo *)ctx, w
 
 void rr_draw(const rr_state *st, int num_verts, uint32_t first, int indexed, rr_counters *ctr)
 {
+  atomic_fetch_add(&tc_generation, 1);
   fifo q = {0};
   rr_vertex *sv;
   int32_t *win;
@@ -1111,6 +1120,7 @@ static void emit_threaded(void *ctx, const tri_work *w)
 void rr_draw_threaded(const rr_state *st, int num_verts, uint32_t first, int indexed,
                       int nthreads, rr_counters *ctr)
 {
+  atomic_fetch_add(&tc_generation, 1);
   pool_start(nthreads);
   memset(&g_pool.ctr, 0, sizeof(g_pool.ctr));
   rr_vertex *sv;
@@ -1150,4 +1160,11 @@ void rr_draw_threaded(const rr_state *st, int num_verts, uint32_t first, int ind
   }
   free(sv);
   free(win);
+}
+
+void rr_sample(const rr_image *img, const rr_sampler *s, const float coord[3], const float ddx[2],
+               const float ddy[2], int use_lod, float out[4])
+{
+  atomic_fetch_add(&tc_generation, 1);
+  sample_full(img, s, coord, ddx, ddy, use_lod, out);
 }

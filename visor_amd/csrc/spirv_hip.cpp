@@ -622,6 +622,8 @@ bool Translator::Parse()
     }
     i += in.wc;
   }
+  if(curf)
+    return Fail("function %" + std::to_string(curf->id) + " has no OpFunctionEnd");
   if(!entry_fn)
     return Fail("entry point '" + entry_name + "' not found for this stage");
   return true;
