@@ -44,7 +44,7 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--plain-steps", type=int, default=3)
-    ap.add_argument("--tile", type=int, default=16)
+    ap.add_argument("--tile", type=int, default=8)
     ap.add_argument("--cpu-draws", type=int, default=128, help="draws in the CPU-baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")

@@ -63,7 +63,7 @@ int vh_device_available(void);
 const char *vh_last_error(void);
 /* 0 = plain per-draw execution (host-synchronised per draw), 1 = batched (default). */
 int vh_set_exec_mode(VkQueue queue, int mode);
-/* Screen tile side for binning and rasterization: 8, 16 (default) or 32. */
+/* Screen tile side for binning and rasterization: 8 (default), 16 or 32. */
 int vh_set_tile_size(VkQueue queue, int tile);
 /* 0 = vkQueueSubmit returns after the GPU finished (the reference's behaviour), 1 = returns at
  * once; completion through fences / vkQueueWaitIdle. */

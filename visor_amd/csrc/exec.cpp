@@ -34,10 +34,10 @@ static uint32_t TileShiftFromEnv()
   if(t)
   {
     int v = atoi(t);
-    if(v == 8) return 3;
+    if(v == 16) return 4;
     if(v == 32) return 5;
   }
-  return 4;
+  return 3;
 }
 
 Queue::Queue()

@@ -1097,131 +1097,149 @@ VKAPI_ATTR VkResult VKAPI_CALL vkQueuePresentKHR(VkQueue queue, const VkPresentI
 // ---------------------------------------------------------------------------------------------
 // loader interface (reference icd_interface.cpp:3-169)
 
-#define ICD_FUNC(name)          \
-  if(!strcmp(pName, #name))     \
-    return (PFN_vkVoidFunction)&name;
+// Entry points by name, sorted for binary search. Every name is visible through
+// vk_icdGetInstanceProcAddr; the scope bits say which names vk_icdGetPhysicalDeviceProcAddr and
+// vkGetDeviceProcAddr also return (the three lists of icd_interface.cpp:7-102).
+enum : uint8_t
+{
+  kInst = 0,
+  kPhys = 1,
+  kDev = 2,
+};
 
-#define CHECK_INST_FUNCS()                          \
-  ICD_FUNC(vkEnumerateInstanceExtensionProperties); \
-  ICD_FUNC(vkEnumerateDeviceExtensionProperties);   \
-  ICD_FUNC(vkGetDeviceProcAddr);                    \
-  ICD_FUNC(vkCreateInstance);                       \
-  ICD_FUNC(vkDestroyInstance);                      \
-  ICD_FUNC(vkEnumeratePhysicalDevices);             \
-  ICD_FUNC(vkCreateDevice);                         \
-  ICD_FUNC(vkEnumerateInstanceVersion);
+struct EntryPoint
+{
+  const char *name;
+  PFN_vkVoidFunction fn;
+  uint8_t scope;
+};
 
-#define CHECK_PHYS_FUNCS()                             \
-  ICD_FUNC(vkGetPhysicalDeviceFeatures);               \
-  ICD_FUNC(vkGetPhysicalDeviceProperties);             \
-  ICD_FUNC(vkGetPhysicalDeviceQueueFamilyProperties);  \
-  ICD_FUNC(vkGetPhysicalDeviceMemoryProperties);       \
-  ICD_FUNC(vkGetPhysicalDeviceFormatProperties);       \
-  ICD_FUNC(vkGetPhysicalDeviceSurfaceSupportKHR);      \
-  ICD_FUNC(vkGetPhysicalDeviceSurfaceFormatsKHR);      \
-  ICD_FUNC(vkGetPhysicalDeviceSurfaceCapabilitiesKHR); \
-  ICD_FUNC(vkGetPhysicalDeviceSurfacePresentModesKHR);
+static const EntryPoint kEntryPoints[] = {
+    {"vkAcquireNextImageKHR", reinterpret_cast<PFN_vkVoidFunction>(&vkAcquireNextImageKHR), kDev},
+    {"vkAllocateCommandBuffers", reinterpret_cast<PFN_vkVoidFunction>(&vkAllocateCommandBuffers), kDev},
+    {"vkAllocateDescriptorSets", reinterpret_cast<PFN_vkVoidFunction>(&vkAllocateDescriptorSets), kDev},
+    {"vkAllocateMemory", reinterpret_cast<PFN_vkVoidFunction>(&vkAllocateMemory), kDev},
+    {"vkBeginCommandBuffer", reinterpret_cast<PFN_vkVoidFunction>(&vkBeginCommandBuffer), kDev},
+    {"vkBindBufferMemory", reinterpret_cast<PFN_vkVoidFunction>(&vkBindBufferMemory), kDev},
+    {"vkBindImageMemory", reinterpret_cast<PFN_vkVoidFunction>(&vkBindImageMemory), kDev},
+    {"vkCmdBeginRenderPass", reinterpret_cast<PFN_vkVoidFunction>(&vkCmdBeginRenderPass), kDev},
+    {"vkCmdBindDescriptorSets", reinterpret_cast<PFN_vkVoidFunction>(&vkCmdBindDescriptorSets), kDev},
+    {"vkCmdBindIndexBuffer", reinterpret_cast<PFN_vkVoidFunction>(&vkCmdBindIndexBuffer), kDev},
+    {"vkCmdBindPipeline", reinterpret_cast<PFN_vkVoidFunction>(&vkCmdBindPipeline), kDev},
+    {"vkCmdBindVertexBuffers", reinterpret_cast<PFN_vkVoidFunction>(&vkCmdBindVertexBuffers), kDev},
+    {"vkCmdCopyBuffer", reinterpret_cast<PFN_vkVoidFunction>(&vkCmdCopyBuffer), kDev},
+    {"vkCmdCopyBufferToImage", reinterpret_cast<PFN_vkVoidFunction>(&vkCmdCopyBufferToImage), kDev},
+    {"vkCmdDraw", reinterpret_cast<PFN_vkVoidFunction>(&vkCmdDraw), kDev},
+    {"vkCmdDrawIndexed", reinterpret_cast<PFN_vkVoidFunction>(&vkCmdDrawIndexed), kDev},
+    {"vkCmdEndRenderPass", reinterpret_cast<PFN_vkVoidFunction>(&vkCmdEndRenderPass), kDev},
+    {"vkCmdPipelineBarrier", reinterpret_cast<PFN_vkVoidFunction>(&vkCmdPipelineBarrier), kDev},
+    {"vkCmdPushConstants", reinterpret_cast<PFN_vkVoidFunction>(&vkCmdPushConstants), kDev},
+    {"vkCmdSetScissor", reinterpret_cast<PFN_vkVoidFunction>(&vkCmdSetScissor), kDev},
+    {"vkCmdSetViewport", reinterpret_cast<PFN_vkVoidFunction>(&vkCmdSetViewport), kDev},
+    {"vkCreateBuffer", reinterpret_cast<PFN_vkVoidFunction>(&vkCreateBuffer), kDev},
+    {"vkCreateCommandPool", reinterpret_cast<PFN_vkVoidFunction>(&vkCreateCommandPool), kDev},
+    {"vkCreateComputePipelines", reinterpret_cast<PFN_vkVoidFunction>(&vkCreateComputePipelines), kDev},
+    {"vkCreateDescriptorPool", reinterpret_cast<PFN_vkVoidFunction>(&vkCreateDescriptorPool), kDev},
+    {"vkCreateDescriptorSetLayout", reinterpret_cast<PFN_vkVoidFunction>(&vkCreateDescriptorSetLayout), kDev},
+    {"vkCreateDevice", reinterpret_cast<PFN_vkVoidFunction>(&vkCreateDevice), kInst},
+    {"vkCreateFence", reinterpret_cast<PFN_vkVoidFunction>(&vkCreateFence), kDev},
+    {"vkCreateFramebuffer", reinterpret_cast<PFN_vkVoidFunction>(&vkCreateFramebuffer), kDev},
+    {"vkCreateGraphicsPipelines", reinterpret_cast<PFN_vkVoidFunction>(&vkCreateGraphicsPipelines), kDev},
+    {"vkCreateImage", reinterpret_cast<PFN_vkVoidFunction>(&vkCreateImage), kDev},
+    {"vkCreateImageView", reinterpret_cast<PFN_vkVoidFunction>(&vkCreateImageView), kDev},
+    {"vkCreateInstance", reinterpret_cast<PFN_vkVoidFunction>(&vkCreateInstance), kInst},
+    {"vkCreatePipelineCache", reinterpret_cast<PFN_vkVoidFunction>(&vkCreatePipelineCache), kDev},
+    {"vkCreatePipelineLayout", reinterpret_cast<PFN_vkVoidFunction>(&vkCreatePipelineLayout), kDev},
+    {"vkCreateRenderPass", reinterpret_cast<PFN_vkVoidFunction>(&vkCreateRenderPass), kDev},
+    {"vkCreateSampler", reinterpret_cast<PFN_vkVoidFunction>(&vkCreateSampler), kDev},
+    {"vkCreateSemaphore", reinterpret_cast<PFN_vkVoidFunction>(&vkCreateSemaphore), kDev},
+    {"vkCreateShaderModule", reinterpret_cast<PFN_vkVoidFunction>(&vkCreateShaderModule), kDev},
+    {"vkCreateSwapchainKHR", reinterpret_cast<PFN_vkVoidFunction>(&vkCreateSwapchainKHR), kDev},
+    {"vkDestroyBuffer", reinterpret_cast<PFN_vkVoidFunction>(&vkDestroyBuffer), kDev},
+    {"vkDestroyCommandPool", reinterpret_cast<PFN_vkVoidFunction>(&vkDestroyCommandPool), kDev},
+    {"vkDestroyDescriptorPool", reinterpret_cast<PFN_vkVoidFunction>(&vkDestroyDescriptorPool), kDev},
+    {"vkDestroyDescriptorSetLayout", reinterpret_cast<PFN_vkVoidFunction>(&vkDestroyDescriptorSetLayout), kDev},
+    {"vkDestroyDevice", reinterpret_cast<PFN_vkVoidFunction>(&vkDestroyDevice), kDev},
+    {"vkDestroyFence", reinterpret_cast<PFN_vkVoidFunction>(&vkDestroyFence), kDev},
+    {"vkDestroyFramebuffer", reinterpret_cast<PFN_vkVoidFunction>(&vkDestroyFramebuffer), kDev},
+    {"vkDestroyImage", reinterpret_cast<PFN_vkVoidFunction>(&vkDestroyImage), kDev},
+    {"vkDestroyImageView", reinterpret_cast<PFN_vkVoidFunction>(&vkDestroyImageView), kDev},
+    {"vkDestroyInstance", reinterpret_cast<PFN_vkVoidFunction>(&vkDestroyInstance), kInst},
+    {"vkDestroyPipeline", reinterpret_cast<PFN_vkVoidFunction>(&vkDestroyPipeline), kDev},
+    {"vkDestroyPipelineCache", reinterpret_cast<PFN_vkVoidFunction>(&vkDestroyPipelineCache), kDev},
+    {"vkDestroyPipelineLayout", reinterpret_cast<PFN_vkVoidFunction>(&vkDestroyPipelineLayout), kDev},
+    {"vkDestroyRenderPass", reinterpret_cast<PFN_vkVoidFunction>(&vkDestroyRenderPass), kDev},
+    {"vkDestroySampler", reinterpret_cast<PFN_vkVoidFunction>(&vkDestroySampler), kDev},
+    {"vkDestroySemaphore", reinterpret_cast<PFN_vkVoidFunction>(&vkDestroySemaphore), kDev},
+    {"vkDestroyShaderModule", reinterpret_cast<PFN_vkVoidFunction>(&vkDestroyShaderModule), kDev},
+    {"vkDestroySwapchainKHR", reinterpret_cast<PFN_vkVoidFunction>(&vkDestroySwapchainKHR), kDev},
+    {"vkDeviceWaitIdle", reinterpret_cast<PFN_vkVoidFunction>(&vkDeviceWaitIdle), kDev},
+    {"vkEndCommandBuffer", reinterpret_cast<PFN_vkVoidFunction>(&vkEndCommandBuffer), kDev},
+    {"vkEnumerateDeviceExtensionProperties", reinterpret_cast<PFN_vkVoidFunction>(&vkEnumerateDeviceExtensionProperties), kInst},
+    {"vkEnumerateInstanceExtensionProperties", reinterpret_cast<PFN_vkVoidFunction>(&vkEnumerateInstanceExtensionProperties), kInst},
+    {"vkEnumerateInstanceVersion", reinterpret_cast<PFN_vkVoidFunction>(&vkEnumerateInstanceVersion), kInst},
+    {"vkEnumeratePhysicalDevices", reinterpret_cast<PFN_vkVoidFunction>(&vkEnumeratePhysicalDevices), kInst},
+    {"vkFlushMappedMemoryRanges", reinterpret_cast<PFN_vkVoidFunction>(&vkFlushMappedMemoryRanges), kDev},
+    {"vkFreeCommandBuffers", reinterpret_cast<PFN_vkVoidFunction>(&vkFreeCommandBuffers), kDev},
+    {"vkFreeDescriptorSets", reinterpret_cast<PFN_vkVoidFunction>(&vkFreeDescriptorSets), kDev},
+    {"vkFreeMemory", reinterpret_cast<PFN_vkVoidFunction>(&vkFreeMemory), kDev},
+    {"vkGetBufferMemoryRequirements", reinterpret_cast<PFN_vkVoidFunction>(&vkGetBufferMemoryRequirements), kDev},
+    {"vkGetDeviceProcAddr", reinterpret_cast<PFN_vkVoidFunction>(&vkGetDeviceProcAddr), kInst},
+    {"vkGetDeviceQueue", reinterpret_cast<PFN_vkVoidFunction>(&vkGetDeviceQueue), kDev},
+    {"vkGetImageMemoryRequirements", reinterpret_cast<PFN_vkVoidFunction>(&vkGetImageMemoryRequirements), kDev},
+    {"vkGetImageSubresourceLayout", reinterpret_cast<PFN_vkVoidFunction>(&vkGetImageSubresourceLayout), kDev},
+    {"vkGetPhysicalDeviceFeatures", reinterpret_cast<PFN_vkVoidFunction>(&vkGetPhysicalDeviceFeatures), kPhys},
+    {"vkGetPhysicalDeviceFormatProperties", reinterpret_cast<PFN_vkVoidFunction>(&vkGetPhysicalDeviceFormatProperties), kPhys},
+    {"vkGetPhysicalDeviceMemoryProperties", reinterpret_cast<PFN_vkVoidFunction>(&vkGetPhysicalDeviceMemoryProperties), kPhys},
+    {"vkGetPhysicalDeviceProperties", reinterpret_cast<PFN_vkVoidFunction>(&vkGetPhysicalDeviceProperties), kPhys},
+    {"vkGetPhysicalDeviceQueueFamilyProperties", reinterpret_cast<PFN_vkVoidFunction>(&vkGetPhysicalDeviceQueueFamilyProperties), kPhys},
+    {"vkGetPhysicalDeviceSurfaceCapabilitiesKHR", reinterpret_cast<PFN_vkVoidFunction>(&vkGetPhysicalDeviceSurfaceCapabilitiesKHR), kPhys},
+    {"vkGetPhysicalDeviceSurfaceFormatsKHR", reinterpret_cast<PFN_vkVoidFunction>(&vkGetPhysicalDeviceSurfaceFormatsKHR), kPhys},
+    {"vkGetPhysicalDeviceSurfacePresentModesKHR", reinterpret_cast<PFN_vkVoidFunction>(&vkGetPhysicalDeviceSurfacePresentModesKHR), kPhys},
+    {"vkGetPhysicalDeviceSurfaceSupportKHR", reinterpret_cast<PFN_vkVoidFunction>(&vkGetPhysicalDeviceSurfaceSupportKHR), kPhys},
+    {"vkGetSwapchainImagesKHR", reinterpret_cast<PFN_vkVoidFunction>(&vkGetSwapchainImagesKHR), kDev},
+    {"vkMapMemory", reinterpret_cast<PFN_vkVoidFunction>(&vkMapMemory), kDev},
+    {"vkQueuePresentKHR", reinterpret_cast<PFN_vkVoidFunction>(&vkQueuePresentKHR), kDev},
+    {"vkQueueSubmit", reinterpret_cast<PFN_vkVoidFunction>(&vkQueueSubmit), kDev},
+    {"vkQueueWaitIdle", reinterpret_cast<PFN_vkVoidFunction>(&vkQueueWaitIdle), kDev},
+    {"vkResetFences", reinterpret_cast<PFN_vkVoidFunction>(&vkResetFences), kDev},
+    {"vkUnmapMemory", reinterpret_cast<PFN_vkVoidFunction>(&vkUnmapMemory), kDev},
+    {"vkUpdateDescriptorSets", reinterpret_cast<PFN_vkVoidFunction>(&vkUpdateDescriptorSets), kDev},
+    {"vkWaitForFences", reinterpret_cast<PFN_vkVoidFunction>(&vkWaitForFences), kDev},
+};
 
-#define CHECK_DEV_FUNCS()                  \
-  ICD_FUNC(vkDestroyDevice);               \
-  ICD_FUNC(vkGetDeviceQueue);              \
-  ICD_FUNC(vkCreateFence);                 \
-  ICD_FUNC(vkDestroyFence);                \
-  ICD_FUNC(vkCreateSemaphore);             \
-  ICD_FUNC(vkDestroySemaphore);            \
-  ICD_FUNC(vkCreateCommandPool);           \
-  ICD_FUNC(vkDestroyCommandPool);          \
-  ICD_FUNC(vkAllocateCommandBuffers);      \
-  ICD_FUNC(vkFreeCommandBuffers);          \
-  ICD_FUNC(vkBeginCommandBuffer);          \
-  ICD_FUNC(vkEndCommandBuffer);            \
-  ICD_FUNC(vkCreateSwapchainKHR);          \
-  ICD_FUNC(vkDestroySwapchainKHR);         \
-  ICD_FUNC(vkGetSwapchainImagesKHR);       \
-  ICD_FUNC(vkAcquireNextImageKHR);         \
-  ICD_FUNC(vkQueuePresentKHR);             \
-  ICD_FUNC(vkCreateImageView);             \
-  ICD_FUNC(vkDestroyImageView);            \
-  ICD_FUNC(vkCreateImage);                 \
-  ICD_FUNC(vkDestroyImage);                \
-  ICD_FUNC(vkCreateBuffer);                \
-  ICD_FUNC(vkDestroyBuffer);               \
-  ICD_FUNC(vkGetBufferMemoryRequirements); \
-  ICD_FUNC(vkGetImageMemoryRequirements);  \
-  ICD_FUNC(vkBindBufferMemory);            \
-  ICD_FUNC(vkBindImageMemory);             \
-  ICD_FUNC(vkAllocateMemory);              \
-  ICD_FUNC(vkFreeMemory);                  \
-  ICD_FUNC(vkMapMemory);                   \
-  ICD_FUNC(vkUnmapMemory);                 \
-  ICD_FUNC(vkFlushMappedMemoryRanges);     \
-  ICD_FUNC(vkGetImageSubresourceLayout);   \
-  ICD_FUNC(vkCreateSampler);               \
-  ICD_FUNC(vkDestroySampler);              \
-  ICD_FUNC(vkCreateDescriptorSetLayout);   \
-  ICD_FUNC(vkDestroyDescriptorSetLayout);  \
-  ICD_FUNC(vkCreatePipelineLayout);        \
-  ICD_FUNC(vkDestroyPipelineLayout);       \
-  ICD_FUNC(vkCreateFramebuffer);           \
-  ICD_FUNC(vkDestroyFramebuffer);          \
-  ICD_FUNC(vkCreateRenderPass);            \
-  ICD_FUNC(vkDestroyRenderPass);           \
-  ICD_FUNC(vkCreateShaderModule);          \
-  ICD_FUNC(vkDestroyShaderModule);         \
-  ICD_FUNC(vkCreatePipelineCache);         \
-  ICD_FUNC(vkDestroyPipelineCache);        \
-  ICD_FUNC(vkCreateGraphicsPipelines);     \
-  ICD_FUNC(vkCreateComputePipelines);      \
-  ICD_FUNC(vkDestroyPipeline);             \
-  ICD_FUNC(vkCreateDescriptorPool);        \
-  ICD_FUNC(vkDestroyDescriptorPool);       \
-  ICD_FUNC(vkAllocateDescriptorSets);      \
-  ICD_FUNC(vkFreeDescriptorSets);          \
-  ICD_FUNC(vkUpdateDescriptorSets);        \
-  ICD_FUNC(vkCmdPipelineBarrier);          \
-  ICD_FUNC(vkCmdBeginRenderPass);          \
-  ICD_FUNC(vkCmdEndRenderPass);            \
-  ICD_FUNC(vkCmdBindIndexBuffer);          \
-  ICD_FUNC(vkCmdBindVertexBuffers);        \
-  ICD_FUNC(vkCmdBindPipeline);             \
-  ICD_FUNC(vkCmdBindDescriptorSets);       \
-  ICD_FUNC(vkCmdSetViewport);              \
-  ICD_FUNC(vkCmdSetScissor);               \
-  ICD_FUNC(vkCmdPushConstants);            \
-  ICD_FUNC(vkCmdDraw);                     \
-  ICD_FUNC(vkCmdDrawIndexed);              \
-  ICD_FUNC(vkCmdCopyBufferToImage);        \
-  ICD_FUNC(vkCmdCopyBuffer);               \
-  ICD_FUNC(vkQueueSubmit);                 \
-  ICD_FUNC(vkWaitForFences);               \
-  ICD_FUNC(vkResetFences);                 \
-  ICD_FUNC(vkDeviceWaitIdle);              \
-  ICD_FUNC(vkQueueWaitIdle);
+static PFN_vkVoidFunction LookupEntryPoint(const char *name, uint8_t need)
+{
+  if(!name)
+    return nullptr;
+  const EntryPoint *lo = kEntryPoints, *hi = kEntryPoints + sizeof(kEntryPoints) / sizeof(kEntryPoints[0]);
+  while(lo < hi)
+  {
+    const EntryPoint *mid = lo + (hi - lo) / 2;
+    int c = strcmp(mid->name, name);
+    if(c == 0)
+      return (mid->scope & need) == need ? mid->fn : nullptr;
+    if(c < 0)
+      lo = mid + 1;
+    else
+      hi = mid;
+  }
+  return nullptr;  // unknown names are NULL (the reference returns strdup(name), :119-122)
+}
 
 VKAPI_ATTR PFN_vkVoidFunction VKAPI_CALL vkGetDeviceProcAddr(VkDevice, const char *pName)
 {
-  if(!pName)
-    return nullptr;
-  CHECK_DEV_FUNCS();
-  return nullptr;  // unknown names return NULL (the reference returns strdup(name), :119-122)
+  return LookupEntryPoint(pName, kDev);
 }
 
 VKAPI_ATTR PFN_vkVoidFunction VKAPI_CALL vk_icdGetInstanceProcAddr(VkInstance, const char *pName)
 {
-  if(!pName)
-    return nullptr;
-  CHECK_INST_FUNCS();
-  CHECK_PHYS_FUNCS();
-  CHECK_DEV_FUNCS();
-  return nullptr;
+  return LookupEntryPoint(pName, kInst);
 }
 
 VKAPI_ATTR PFN_vkVoidFunction VKAPI_CALL vk_icdGetPhysicalDeviceProcAddr(VkInstance, const char *pName)
 {
-  if(!pName)
-    return nullptr;
-  CHECK_PHYS_FUNCS();
-  return nullptr;
+  return LookupEntryPoint(pName, kPhys);
 }
 
 VKAPI_ATTR VkResult VKAPI_CALL vk_icdNegotiateLoaderICDInterfaceVersion(uint32_t *pSupportedVersion)
