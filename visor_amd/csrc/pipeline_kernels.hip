@@ -250,11 +250,6 @@ VA_INLINE u32 va_unorm_out(float v)
   return (u32)(v * 255.0f);
 }
 
-VA_INLINE float va_blend_factor(u32 f, float a)
-{
-  return f == 0u ? 0.0f : (f == 6u ? a : (f == 7u ? 1.0f - a : 1.0f));
-}
-
 VA_INLINE u32 va_half_bits(float f) { _Float16 h = (_Float16)f; return (u32)__builtin_bit_cast(u16, h); }
 VA_INLINE float va_half_val(u32 b) { return (float)__builtin_bit_cast(_Float16, (u16)b); }
 
@@ -286,18 +281,35 @@ VA_INLINE bool va_bary(const u32 *__restrict__ r, i32 x, i32 y, float b[4], floa
   return covered;
 }
 
+// Fixed-function state of a record's draw in 9 bits: depth op (0-2), depth write (3), blend
+// enable (4), source and destination colour factor codes (5-6, 7-8: 0 ZERO, 1 ONE and every
+// other factor, 2 SRC_ALPHA, 3 ONE_MINUS_SRC_ALPHA; reference rasterizer.cpp:600-640).
+#define VA_F_DWRITE 8u
+#define VA_F_BLEND 16u
+VA_INLINE u32 va_factor_code(u32 f) { return f == 0u ? 0u : (f == 6u ? 2u : (f == 7u ? 3u : 1u)); }
+VA_INLINE u16 va_rec_flags(const VaDraw &d)
+{
+  return (u16)((d.depth_op & 7u) | (d.depth_write ? VA_F_DWRITE : 0u) | (d.blend_enable ? VA_F_BLEND : 0u) |
+               (va_factor_code(d.src_factor) << 5) | (va_factor_code(d.dst_factor) << 7));
+}
+VA_INLINE float va_factor_value(u32 code, float a)
+{
+  return code == 0u ? 0.0f : (code == 2u ? a : (code == 3u ? 1.0f - a : 1.0f));
+}
+
 // Blend (rasterizer.cpp:593-672) and store (:674-676) one shaded fragment into the pixel's
 // registers. 8-bit targets: dst = (byte[2], byte[1], byte[0]) / 255 with dst alpha 1, factors
 // from the colour factors, bytes B,G,R truncated, alpha byte untouched. RGBA16F targets (beyond
 // the reference) blend all four channels and store halves.
-VA_INLINE void va_blend_store(const VaDraw &d, bool hdr, float o[4], u32 &c0, u32 &c1)
+VA_INLINE void va_blend_store_flags(u32 f, bool hdr, float o[4], u32 &c0, u32 &c1)
 {
+  const bool blend = (f & VA_F_BLEND) != 0u;
   if(hdr)
   {
-    if(d.blend_enable)
+    if(blend)
     {
-      const float sf = va_blend_factor(d.src_factor, o[3]);
-      const float df_ = va_blend_factor(d.dst_factor, o[3]);
+      const float sf = va_factor_value((f >> 5) & 3u, o[3]);
+      const float df_ = va_factor_value((f >> 7) & 3u, o[3]);
       const float e0 = va_half_val(c0 & 0xffffu), e1 = va_half_val(c0 >> 16);
       const float e2 = va_half_val(c1 & 0xffffu), e3 = va_half_val(c1 >> 16);
       o[0] = sf * o[0] + df_ * e0;
@@ -310,22 +322,24 @@ VA_INLINE void va_blend_store(const VaDraw &d, bool hdr, float o[4], u32 &c0, u3
   }
   else
   {
-    if(d.blend_enable)
+    if(blend)
     {
-      const float sf = va_blend_factor(d.src_factor, o[3]);
-      const float df_ = va_blend_factor(d.dst_factor, o[3]);
+      const float sf = va_factor_value((f >> 5) & 3u, o[3]);
+      const float df_ = va_factor_value((f >> 7) & 3u, o[3]);
       const float e0 = va_unorm_in((c0 >> 16) & 0xffu);
       const float e1 = va_unorm_in((c0 >> 8) & 0xffu);
       const float e2 = va_unorm_in(c0 & 0xffu);
-      const float r0 = sf * o[0] + df_ * e0;
-      const float r1 = sf * o[1] + df_ * e1;
-      const float r2 = sf * o[2] + df_ * e2;
-      o[0] = r0;
-      o[1] = r1;
-      o[2] = r2;
+      o[0] = sf * o[0] + df_ * e0;
+      o[1] = sf * o[1] + df_ * e1;
+      o[2] = sf * o[2] + df_ * e2;
     }
     c0 = (c0 & 0xff000000u) | (va_unorm_out(o[0]) << 16) | (va_unorm_out(o[1]) << 8) | va_unorm_out(o[2]);
   }
+}
+
+VA_INLINE void va_blend_store(const VaDraw &d, bool hdr, float o[4], u32 &c0, u32 &c1)
+{
+  va_blend_store_flags(va_rec_flags(d), hdr, o, c0, c1);
 }
 
 // depth compare (rasterizer.cpp:562-576)
@@ -344,11 +358,6 @@ VA_INLINE bool va_depth_pass(u32 op, float z, float cur)
   }
 }
 
-VA_INLINE u32 va_lanes_below(u64 m)
-{
-  return __builtin_amdgcn_mbcnt_hi((u32)(m >> 32), __builtin_amdgcn_mbcnt_lo((u32)m, 0u));
-}
-
 // Orders this wave's LDS writes before its later LDS reads by other lanes.
 VA_INLINE void va_wave_sync()
 {
@@ -358,19 +367,80 @@ VA_INLINE void va_wave_sync()
 }
 
 #ifndef VA_CHUNK
-#define VA_CHUNK 64u       // bin entries staged in LDS per round
+#define VA_CHUNK 64u       // bin entries staged in LDS per round (at most 64: one bit each)
 #endif
-#define VA_MAXJOBS 128u    // pending fragments per wave before a shading flush
 #define VA_SMALL 16        // bbox-in-block overlaps up to this many pixels are tested per triangle
+static_assert(VA_CHUNK <= 64u, "record sets are 64-bit masks");
+
+// Transposes the 64x64 bit matrix held one row per lane (bit j of lane i = element (i, j)):
+// afterwards bit i of lane j is the former element (i, j). Six block-swap steps, each exchanging
+// the off-diagonal s x s blocks of every 2s x 2s block with the partner lane (lane ^ s).
+VA_INLINE u64 va_transpose64(u64 x, u32 lane)
+{
+  const u64 masks[6] = {0x00000000FFFFFFFFull, 0x0000FFFF0000FFFFull, 0x00FF00FF00FF00FFull,
+                        0x0F0F0F0F0F0F0F0Full, 0x3333333333333333ull, 0x5555555555555555ull};
+#pragma unroll
+  for(int i = 0; i < 6; i++)
+  {
+    const u32 s = 32u >> i;
+    const u64 m = masks[i];
+    const u32 ylo = (u32)__shfl_xor((int)(u32)x, (int)s);
+    const u32 yhi = (u32)__shfl_xor((int)(u32)(x >> 32), (int)s);
+    const u64 y = ((u64)yhi << 32) | ylo;
+    x = (lane & s) ? ((x & ~m) | ((y >> s) & m)) : ((x & m) | ((y << s) & ~m));
+  }
+  return x;
+}
+
+VA_INLINE u32 va_wave_exclusive_sum(u32 v, u32 lane)
+{
+  u32 incl = v;
+#pragma unroll
+  for(u32 o = 1u; o < 64u; o <<= 1)
+  {
+    const u32 t = (u32)__shfl_up((int)incl, o);
+    if(lane >= o)
+      incl += t;
+  }
+  return incl - v;
+}
+
+// Position of the n-th (0-based) set bit of m; n < popcount(m).
+VA_INLINE u32 va_nth_set_bit(u64 m, u32 n)
+{
+  u32 pos = 0u, w = (u32)m, c = (u32)__popc(w);
+  if(n >= c) { n -= c; pos = 32u; w = (u32)(m >> 32); }
+  c = (u32)__popc(w & 0xffffu);
+  if(n >= c) { n -= c; pos += 16u; w >>= 16; }
+  c = (u32)__popc(w & 0xffu);
+  if(n >= c) { n -= c; pos += 8u; w >>= 8; }
+  c = (u32)__popc(w & 0xfu);
+  if(n >= c) { n -= c; pos += 4u; w >>= 4; }
+  c = (u32)__popc(w & 0x3u);
+  if(n >= c) { n -= c; pos += 2u; w >>= 2; }
+  return pos + ((n >= (w & 1u)) ? 1u : 0u);
+}
+
+// Coverage only (the integer part of va_bary).
+VA_INLINE bool va_covered(const u32 *__restrict__ r, i32 x, i32 y)
+{
+  const i32 PAx = (i32)r[VA_R_X0] - x;
+  const i32 PAy = (i32)r[VA_R_Y0] - y;
+  const u32 ux = r[VA_R_ACX] * (u32)PAy - r[VA_R_ACY] * (u32)PAx;
+  const u32 uy = (u32)PAx * r[VA_R_ABY] - (u32)PAy * r[VA_R_ABX];
+  return (i32)(r[VA_R_AREA] - (ux + uy)) >= 0 && (i32)ux >= 0 && (i32)uy >= 0;
+}
 
 // One workgroup per tile; wave w owns the 8x8 block w of the tile, one pixel per lane, colour
-// and depth in registers. The workgroup stages VA_CHUNK bin entries' triangle records in LDS;
-// each wave then walks them in bin (API primitive) order: bbox reject, integer edge functions,
-// early depth test and depth write in order per pixel. Fragments that pass are appended to a
-// wave-local job list; a flush shades up to 64 of them per step regardless of which triangle or
-// pixel they belong to, then every pixel blends its own results in primitive order. The
-// arithmetic and order of every fragment are those of the reference; only the lane that runs
-// a fragment program changes.
+// and depth in registers from first touch to the final store. The workgroup stages VA_CHUNK bin
+// entries' triangle records in LDS (bin order = API primitive order). Each wave then builds the
+// coverage matrix of the chunk over its block (triangle-parallel for small overlaps,
+// pixel-parallel for large ones), transposes it so each lane holds the ordered set of triangles
+// covering its pixel, and runs the early depth test and depth write per pixel in primitive order.
+// The fragments that pass are shaded 64 per step regardless of which triangle or pixel they
+// belong to, and each pixel blends its own results in primitive order. The arithmetic and order
+// of every fragment are those of the reference; only the lane that runs a fragment program
+// changes.
 // Coverage and interpolated depth only (the first half of va_bary, same operations).
 VA_INLINE bool va_cov_depth(const u32 *__restrict__ r, i32 x, i32 y, float *depth)
 {
@@ -398,10 +468,10 @@ VA_INLINE void va_raster_tile(const VaDraw *__restrict__ draws, const VaFrame *_
   constexpr u32 Q = VA_REC_STRIDE / 4u;
   __shared__ uint4 s_rec4[VA_CHUNK * Q];
   __shared__ u32 s_tri[VA_CHUNK];
-  __shared__ u64 s_mask[NW][VA_CHUNK];
-  __shared__ u16 s_base[NW][VA_CHUNK];
-  __shared__ u16 s_job[NW][VA_MAXJOBS];
-  __shared__ float4 s_res[NW][VA_MAXJOBS];
+  __shared__ u16 s_flags[VA_CHUNK];   // per-record fixed-function state (va_rec_flags)
+  __shared__ u64 s_pass[NW][64];      // per pixel: records of the chunk that passed depth
+  __shared__ u32 s_off[NW][64];       // per pixel: first index in the wave's job order
+  __shared__ float4 s_res[NW][64];    // fragment program results of one shading round
   const u32 *s_rec = (const u32 *)s_rec4;
 
   const VaFrame &fr = frames[va_find_frame(frames, nframes, tile)];
@@ -469,6 +539,9 @@ VA_INLINE void va_raster_tile(const VaDraw *__restrict__ draws, const VaFrame *_
       s_rec4[w] = ((const uint4 *)(rec + (u64)s_tri[k] * VA_REC_STRIDE))[q];
     }
     __syncthreads();
+    for(u32 k = threadIdx.x; k < n; k += NT)
+      s_flags[k] = va_rec_flags(draws[s_rec[k * VA_REC_STRIDE + VA_R_DRAW]]);
+    __syncthreads();
 
 #if VA_FS_KILL
     // fragment programs that may discard decide depth writes: shade in place, in order
@@ -501,46 +574,12 @@ VA_INLINE void va_raster_tile(const VaDraw *__restrict__ draws, const VaFrame *_
       }
     }
 #else
-    u32 njobs = 0, k0 = 0;
-    auto flush = [&](u32 ka, u32 kb) {
-      va_wave_sync();
-      for(u32 j = lane; j < njobs; j += 64u)
-      {
-        const u32 job = s_job[wave][j];
-        const u32 k = job >> 6, pl = job & 63u;
-        const u32 *r = s_rec + k * VA_REC_STRIDE;
-        const i32 x = bx + (i32)(pl & 7u), y = by + (i32)(pl >> 3);
-        float b[3][4];
-        va_bary(r, x, y, b[0], nullptr);
-#if VA_FS_DERIV
-        va_bary(r, x + 1, y, b[1], nullptr);
-        va_bary(r, x, y + 1, b[2], nullptr);
-#endif
-        const VaDraw &d = draws[r[VA_R_DRAW]];
-        float o[4];
-        va_fs(d.slot, d, (const float *)(r + VA_REC_HDR), b, o);
-        s_res[wave][j] = make_float4(o[0], o[1], o[2], o[3]);
-      }
-      va_wave_sync();
-      for(u32 k = ka; k < kb; ++k)
-      {
-        const u64 m = s_mask[wave][k];
-        if(!m) continue;
-        if((m >> lane) & 1ull)
-        {
-          const float4 v = s_res[wave][s_base[wave][k] + va_lanes_below(m)];
-          float o[4] = {v.x, v.y, v.z, v.w};
-          const VaDraw &d = draws[s_rec[k * VA_REC_STRIDE + VA_R_DRAW]];
-          va_blend_store(d, hdr, o, c0, c1);
-        }
-      }
-      va_wave_sync();
-    };
-    // Triangle-parallel pre-pass: lane k tests triangle k's bbox against this wave's block and,
-    // when the overlap is at most VA_SMALL pixels, evaluates the edge functions there itself
-    // (same integer arithmetic as the pixel-parallel test), giving a 64-bit pixel mask. Larger
-    // overlaps are marked "big" and tested pixel-parallel in the ordered pass below.
-    u32 cm_lo = 0u, cm_hi = 0u;
+    // 1. Coverage, triangle-parallel where cheap: lane k tests triangle k against this wave's
+    //    block when the bbox overlap is at most VA_SMALL pixels (same integer arithmetic as the
+    //    pixel-parallel test); larger overlaps are tested pixel-parallel, one ballot per
+    //    triangle. Either way lane k ends up holding row k of the coverage matrix
+    //    (bit p = pixel p covered by triangle k).
+    u64 row_mask = 0ull;
     bool big = false;
     if(lane < n)
     {
@@ -564,80 +603,105 @@ VA_INLINE void va_raster_tile(const VaDraw *__restrict__ draws, const VaFrame *_
               const u32 ux = ACX * (u32)PAy - ACY * (u32)PAx;
               const u32 uy = (u32)PAx * ABY - (u32)PAy * ABX;
               if((i32)(AR - (ux + uy)) >= 0 && (i32)ux >= 0 && (i32)uy >= 0)
-              {
-                const u32 bit = (u32)((y - by) * 8 + (x - bx));
-                if(bit < 32u) cm_lo |= 1u << bit; else cm_hi |= 1u << (bit - 32u);
-              }
+                row_mask |= 1ull << (u32)((y - by) * 8 + (x - bx));
             }
         }
       }
-      s_mask[wave][lane] = 0ull;
     }
-    const u64 bigmask = __ballot(big);
-    u64 todo = __ballot((cm_lo | cm_hi) != 0u) | bigmask;
-    va_wave_sync();
-    // Ordered pass (primitive order): early depth test and depth write per pixel, job append.
-    while(todo)
+    for(u64 bigs = __ballot(big); bigs; bigs &= bigs - 1ull)
+    {
+      const u32 k = (u32)__builtin_ctzll(bigs);
+      const u32 *r = s_rec + k * VA_REC_STRIDE;
+      const u32 bmin = r[VA_R_BBMIN], bmax = r[VA_R_BBMAX];
+      const bool cov = inside && va_covered(r, px, py) && px >= (i32)(bmin & 0xffffu) && px < (i32)(bmax & 0xffffu) &&
+                       py >= (i32)(bmin >> 16) && py < (i32)(bmax >> 16);
+      const u64 m = __ballot(cov);
+      if(lane == k)
+        row_mask = m;
+    }
+    // 2. Transpose: lane p now holds the set of triangles (bit k) covering pixel p, so the
+    //    in-order loop below runs once per fragment of this pixel, not once per triangle.
+    u64 cov = va_transpose64(row_mask, lane);
+
+    // 3. Early depth test and depth write per pixel in primitive order (ascending k).
+    u64 pass = 0ull;
+    for(u64 todo = cov; todo; todo &= todo - 1ull)
     {
       const u32 k = (u32)__builtin_ctzll(todo);
-      todo &= todo - 1ull;
-      const u32 *r = s_rec + k * VA_REC_STRIDE;
-      float pixdepth;
-      bool cov;
-      if((bigmask >> k) & 1ull)
+      const u32 f = s_flags[k];
+      const u32 op = f & 7u;
+      n_cov++;
+      bool passed = true;
+      float z = 0.0f;
+      if(has_depth && (op != 7u || (f & VA_F_DWRITE)))
       {
-        const u32 bmin = r[VA_R_BBMIN], bmax = r[VA_R_BBMAX];
-        cov = va_cov_depth(r, px, py, &pixdepth);
-        cov = cov && inside && px >= (i32)(bmin & 0xffffu) && px < (i32)(bmax & 0xffffu) && py >= (i32)(bmin >> 16) &&
-              py < (i32)(bmax >> 16);
-      }
-      else
-      {
-        const u32 lo = (u32)__builtin_amdgcn_readlane((int)cm_lo, (int)k);
-        const u32 hi = (u32)__builtin_amdgcn_readlane((int)cm_hi, (int)k);
-        cov = ((lane < 32u ? lo >> lane : hi >> (lane - 32u)) & 1u) != 0u;
-        if(cov)
-          va_cov_depth(r, px, py, &pixdepth);
-      }
-      if(__ballot(cov) == 0ull)
-        continue;
-      const VaDraw &d = draws[r[VA_R_DRAW]];
-      bool passed = cov;
-      if(cov)
-      {
-        n_cov++;
-        if(d.depth_op != 7u && has_depth)
-          passed = va_depth_pass(d.depth_op, pixdepth, dep);
-      }
-      const u64 m = __ballot(passed);
-      if(!m)
-        continue;
-      const u32 cnt = (u32)__popcll(m);
-      if(njobs + cnt > VA_MAXJOBS)
-      {
-        flush(k0, k);
-        njobs = 0;
-        k0 = k;
+        va_cov_depth(s_rec + k * VA_REC_STRIDE, px, py, &z);
+        if(op != 7u)
+          passed = va_depth_pass(op, z, dep);
       }
       if(passed)
       {
-        s_job[wave][njobs + va_lanes_below(m)] = (u16)((k << 6) | lane);
+        pass |= 1ull << k;
         n_pass++;
-        n_blend += d.blend_enable ? 1u : 0u;
-        if(d.depth_write && has_depth)
+        n_blend += (f >> 4) & 1u;
+        if((f & VA_F_DWRITE) && has_depth)
         {
-          dep = pixdepth;
+          dep = z;
           n_dw++;
         }
       }
-      if(lane == 0u)
-      {
-        s_base[wave][k] = (u16)njobs;
-        s_mask[wave][k] = m;
-      }
-      njobs += cnt;
     }
-    flush(k0, n);
+
+    // 4. Shade every passing fragment in rounds of 64, jobs in pixel-major order (pixel p owns
+    //    jobs [off_p, off_p + count_p), in ascending k), then each pixel blends its own results
+    //    of the round in order.
+    const u32 cnt = (u32)__popcll(pass);
+    const u32 off = va_wave_exclusive_sum(cnt, lane);
+    const u32 total = (u32)__builtin_amdgcn_readlane((int)(off + cnt), 63);
+    s_pass[wave][lane] = pass;
+    s_off[wave][lane] = off;
+    u64 rem = pass;
+    u32 next = off;
+    for(u32 base = 0; base < total; base += 64u)
+    {
+      va_wave_sync();
+      const u32 j = base + lane;
+      if(j < total)
+      {
+        // pixel = last p with off_p <= j (pixels without jobs share the next pixel's offset)
+        // (p + step never exceeds 63: the steps sum to 63)
+        u32 p = 0;
+#pragma unroll
+        for(u32 step = 32u; step > 0u; step >>= 1)
+          if(s_off[wave][p + step] <= j)
+            p += step;
+        // k < n by construction; the clamp keeps a record of this chunk under any mistake
+        const u32 k = min(va_nth_set_bit(s_pass[wave][p], j - s_off[wave][p]), n - 1u);
+        const u32 *r = s_rec + k * VA_REC_STRIDE;
+        const i32 x = bx + (i32)(p & 7u), y = by + (i32)(p >> 3);
+        float b[3][4];
+        va_bary(r, x, y, b[0], nullptr);
+#if VA_FS_DERIV
+        va_bary(r, x + 1, y, b[1], nullptr);
+        va_bary(r, x, y + 1, b[2], nullptr);
+#endif
+        const VaDraw &d = draws[r[VA_R_DRAW]];
+        float o[4];
+        va_fs(d.slot, d, (const float *)(r + VA_REC_HDR), b, o);
+        s_res[wave][lane] = make_float4(o[0], o[1], o[2], o[3]);
+      }
+      va_wave_sync();
+      const u32 stop = min(off + cnt, base + 64u);
+      for(; next < stop; ++next)
+      {
+        const u32 k = (u32)__builtin_ctzll(rem);
+        rem &= rem - 1ull;
+        const float4 v = s_res[wave][next - base];
+        float o[4] = {v.x, v.y, v.z, v.w};
+        va_blend_store_flags(s_flags[k], hdr, o, c0, c1);
+      }
+    }
+    va_wave_sync();
 #endif
   }
 
