@@ -38,6 +38,19 @@ def raster_bytes(st, width, height, color_bytes, texel_bytes):
             + width * height * (color_bytes + 4))
 
 
+def profiled_traffic(tile):
+    """HBM bytes per raster launch from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary of
+    this workload and tile size (null when no such summary exists)."""
+    path = os.path.join(ROOT, "profiles", "va_raster_%d__256draws_1080p.md" % tile)
+    try:
+        for line in open(path):
+            if line.startswith("traffic_bytes_per_launch:"):
+                return int(line.split(":")[1])
+    except OSError:
+        pass
+    return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -108,7 +121,7 @@ def main():
     b_raster = raster_bytes(stats_one, scene.width, scene.height, 4, 32)
     achieved = b_raster / (raster_ms * 1e-3) / 1e9
     roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None}
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": profiled_traffic(args.tile)}
 
     # plain per-draw path
     plain = None
