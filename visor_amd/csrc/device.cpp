@@ -197,7 +197,10 @@ static const char *kRtcOptions[] = {
 };
 
 // Assembles the module source for a list of shader pairs (one slot each).
-static std::string ModuleSource(const std::vector<std::shared_ptr<ShaderPair>> &pairs, uint32_t &rec_stride)
+static bool ReadFile(const std::string &path, std::vector<char> &out);
+
+static std::string ModuleSource(const std::vector<std::shared_ptr<ShaderPair>> &pairs, uint32_t &rec_stride,
+                                const TileConfig &cfg)
 {
   uint32_t maxnv = 0;
   bool deriv = false;
@@ -217,6 +220,8 @@ static std::string ModuleSource(const std::vector<std::shared_ptr<ShaderPair>> &
   for(auto &p : pairs)
     kill |= p->fs.has_kill;
   s << "#define VA_FS_KILL " << (kill ? 1 : 0) << "\n";
+  s << "#define VA_CHUNK " << cfg.chunk << "u\n";
+  s << "#define VA_SMALL " << cfg.small << "\n";
   for(auto &p : pairs)
     s << p->vs.code << "\n" << p->fs.code << "\n";
   s << "VA_INLINE void va_vs(u32 slot, const VaDraw &d, u32 vtx, float pos[4], float *vary)\n{\n";
@@ -237,7 +242,13 @@ static std::string ModuleSource(const std::vector<std::shared_ptr<ShaderPair>> &
   for(size_t i = 0; i < pairs.size(); i++)
     s << "    case " << i << "u: return " << pairs[i]->fs.fn_name << "(d, vary, b, out);\n";
   s << "    default: return false;\n  }\n}\n";
-  s << kSrcKernels << "\n";
+  // VISOR_KERNEL_SOURCE: development override of the built-in kernel file (same interface)
+  std::vector<char> override_src;
+  const char *ks = getenv("VISOR_KERNEL_SOURCE");
+  if(ks && ReadFile(ks, override_src))
+    s << std::string(override_src.begin(), override_src.end()) << "\n";
+  else
+    s << kSrcKernels << "\n";
   return s.str();
 }
 
@@ -273,6 +284,12 @@ static bool BuildCodeObject(const std::string &src, const std::string &arch, con
   key = HexKey(Fnv1a(src, Fnv1a(opts_joined)));
   std::string path = cache_dir + "/" + key + ".hsaco";
   seconds = 0.0;
+  if(const char *dump = getenv("VISOR_DUMP_SOURCE"))
+  {
+    // every module's HIP source, for reading the generated code and its ISA
+    std::ofstream f(std::string(dump) + "/" + key + ".hip", std::ios::binary);
+    f << src;
+  }
   if(ReadFile(path, code))
     return true;
   double t0 = NowSeconds();
@@ -313,9 +330,10 @@ static bool BuildCodeObject(const std::string &src, const std::string &arch, con
   return true;
 }
 
-std::shared_ptr<PipelineModule> Device::GetModule(const std::vector<std::shared_ptr<ShaderPair>> &ps, std::string &err)
+std::shared_ptr<PipelineModule> Device::GetModule(const std::vector<std::shared_ptr<ShaderPair>> &ps, std::string &err,
+                                                  const TileConfig &cfg)
 {
-  std::string mkey;
+  std::string mkey = "c" + std::to_string(cfg.chunk) + "s" + std::to_string(cfg.small) + ":";
   for(auto &p : ps)
     mkey += p->key + ",";
   {
@@ -326,7 +344,7 @@ std::shared_ptr<PipelineModule> Device::GetModule(const std::vector<std::shared_
   }
   auto pm = std::make_shared<PipelineModule>();
   pm->key = mkey;
-  std::string src = ModuleSource(ps, pm->rec_stride);
+  std::string src = ModuleSource(ps, pm->rec_stride, cfg);
   std::vector<char> code;
   std::string ckey;
   if(!BuildCodeObject(src, arch, cache_dir, code, ckey, pm->compile_seconds, err))
@@ -365,7 +383,7 @@ bool BuildModuleOffline(const std::vector<std::shared_ptr<ShaderPair>> &ps, cons
                         const std::string &cache_dir, std::string &err, size_t *code_size, std::string *src_out)
 {
   uint32_t stride;
-  std::string src = ModuleSource(ps, stride);
+  std::string src = ModuleSource(ps, stride, TileConfig());
   if(src_out)
     *src_out = src;
   std::vector<char> code;

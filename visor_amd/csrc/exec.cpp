@@ -46,6 +46,19 @@ Queue::Queue()
   if(m && !strcmp(m, "plain"))
     mode = ExecMode::Plain;
   tiles.tile_shift = TileShiftFromEnv();
+  // VISOR_CHUNK / VISOR_SMALL: raster module parameters (see TileConfig)
+  if(const char *c = getenv("VISOR_CHUNK"))
+  {
+    int v = atoi(c);
+    if(v >= 16 && v <= 64)
+      tiles.chunk = (uint32_t)v;
+  }
+  if(const char *c = getenv("VISOR_SMALL"))
+  {
+    int v = atoi(c);
+    if(v >= 0 && v <= 64)
+      tiles.small = (uint32_t)v;
+  }
   const char *a = getenv("VISOR_SUBMIT");
   async = a && !strcmp(a, "async");
 }
@@ -62,7 +75,7 @@ Queue::~Queue()
     worker.join();
   }
   void *bufs[] = {d_frames, d_draws, d_rec, d_count, d_offs, d_keys, d_vals, d_keys2, d_vals2, d_start, d_end, d_ctr, d_temp,
-                  d_blocks, d_part};
+                  d_blocks, d_part, d_stemp};
   for(void *b : bufs)
     if(b)
       hipFree(b);
@@ -602,7 +615,7 @@ VkResult Queue::RunPass(const std::vector<Op *> &passes, bool fused_clear, ExecS
   if(!pairs.empty())
   {
     std::string err;
-    mod = Device::Get().GetModule(pairs, err);
+    mod = Device::Get().GetModule(pairs, err, tiles);
     if(!mod)
     {
       LogError("pipeline module: %s", err.c_str());
@@ -623,12 +636,14 @@ VkResult Queue::RunPass(const std::vector<Op *> &passes, bool fused_clear, ExecS
   }
   const uint32_t stride = mod->rec_stride;
   const uint32_t nframes = (uint32_t)frames.size(), ndraws = (uint32_t)draws.size();
+  const uint32_t nblocks = (uint32_t)block_draw.size();
   if(!Grow(&d_frames, c_frames, sizeof(VaFrame) * nframes) || !Grow(&d_draws, c_draws, sizeof(VaDraw) * std::max(ndraws, 1u)) ||
      !Grow(&d_rec, c_rec, (size_t)stride * 4 * std::max(tri_total, 1u)) || !Grow(&d_count, c_count, 4ull * (tri_total + 1)) ||
      !Grow(&d_offs, c_offs, 4ull * (tri_total + 1)) || !Grow(&d_start, c_start, 4ull * tile_total) ||
      !Grow(&d_end, c_end, 4ull * tile_total) || !Grow(&d_ctr, c_ctr, sizeof(VaCounters)) ||
-     !Grow(&d_blocks, c_blocks, 4ull * std::max<size_t>(block_draw.size(), 1)) ||
-     !Grow(&d_part, c_part, 32ull * (std::max<size_t>(block_draw.size(), tile_total) + 1)))
+     !Grow(&d_blocks, c_blocks, 4ull * std::max<size_t>(nblocks, 1)) ||
+     !Grow(&d_part, c_part, 32ull * (std::max<size_t>(nblocks, tile_total) + 1)) ||
+     !Grow(&d_temp, c_temp, 4 * va_scan_temp_words(tri_total + 1)))
     return VK_ERROR_OUT_OF_DEVICE_MEMORY;
   if(!h_pinned)
     HIPCHK(hipHostMalloc((void **)&h_pinned, 4096, hipHostMallocDefault));
@@ -639,74 +654,85 @@ VkResult Queue::RunPass(const std::vector<Op *> &passes, bool fused_clear, ExecS
   HIPCHK(hipMemcpyAsync(d_frames, frames.data(), sizeof(VaFrame) * nframes, hipMemcpyHostToDevice, stream));
   if(ndraws)
     HIPCHK(hipMemcpyAsync(d_draws, draws.data(), sizeof(VaDraw) * ndraws, hipMemcpyHostToDevice, stream));
-  if(!block_draw.empty())
-    HIPCHK(hipMemcpyAsync(d_blocks, block_draw.data(), 4 * block_draw.size(), hipMemcpyHostToDevice, stream));
-  HIPCHK(va_fill_u32_async(d_ctr, sizeof(VaCounters) / 4, 0u, stream));
-  HIPCHK(hipEventRecord(ev[0], stream));
+  if(nblocks)
+    HIPCHK(hipMemcpyAsync(d_blocks, block_draw.data(), 4 * nblocks, hipMemcpyHostToDevice, stream));
+
+  // The pair buffers are sized by a capacity that only grows: the device work never waits for
+  // the host. If a frame needs more pairs than the capacity, the raster launch writes nothing
+  // and the pass runs again with larger buffers.
+  if(pair_cap == 0)
+    pair_cap = std::max<uint32_t>(1u << 20, 2 * tri_total);
   uint32_t total_pairs = 0;
-  if(tri_total)
+  for(int attempt = 0;; attempt++)
   {
+    if(!Grow(&d_keys, c_keys, 4ull * pair_cap) || !Grow(&d_vals, c_vals, 4ull * pair_cap) ||
+       !Grow(&d_keys2, c_keys2, 4ull * pair_cap) || !Grow(&d_vals2, c_vals2, 4ull * pair_cap) ||
+       !Grow(&d_stemp, c_stemp, 4 * va_sort_temp_words(pair_cap)))
+      return VK_ERROR_OUT_OF_DEVICE_MEMORY;
     VaDraw *dd = (VaDraw *)d_draws;
     VaFrame *df = (VaFrame *)d_frames;
-    uint32_t *rec = (uint32_t *)d_rec, *cnt = (uint32_t *)d_count;
+    uint32_t *rec = (uint32_t *)d_rec, *cnt = (uint32_t *)d_count, *offs = (uint32_t *)d_offs;
+    uint32_t *bd = (uint32_t *)d_blocks, *total_dev = offs + tri_total;
     VaCounters *ctr = (VaCounters *)d_ctr;
-    uint32_t *bd = (uint32_t *)d_blocks;
     uint64_t *part = (uint64_t *)d_part;
-    void *args[] = {&dd, &bd, &df, (void *)&ts, &rec, &cnt, &part};
-    HIPCHK(hipModuleLaunchKernel(mod->setup, (uint32_t)block_draw.size(), 1, 1, 256, 1, 1, 0, stream, args, nullptr));
-    HIPCHK(va_reduce_cols_async(part, (uint32_t)block_draw.size(), 4, (uint64_t *)&ctr->tris_in, stream));
-    HIPCHK(va_fill_u32_async(cnt + tri_total, 1, 0u, stream));
-    size_t need = 0;
-    HIPCHK(va_scan_u32(nullptr, &need, cnt, (uint32_t *)d_offs, tri_total + 1, stream));
-    if(!Grow(&d_temp, c_temp, need))
-      return VK_ERROR_OUT_OF_DEVICE_MEMORY;
-    size_t have = c_temp;
-    HIPCHK(va_scan_u32(d_temp, &have, cnt, (uint32_t *)d_offs, tri_total + 1, stream));
-    HIPCHK(hipMemcpyAsync(h_pinned, (uint32_t *)d_offs + tri_total, 4, hipMemcpyDeviceToHost, stream));
+    uint32_t cap = pair_cap;
+    HIPCHK(va_fill_u32_async(d_ctr, sizeof(VaCounters) / 4, 0u, stream));
+    HIPCHK(hipEventRecord(ev[0], stream));
+    if(tri_total)
+    {
+      void *args[] = {&dd, &bd, &df, (void *)&ts, &rec, &cnt, &part};
+      HIPCHK(hipModuleLaunchKernel(mod->setup, nblocks, 1, 1, 256, 1, 1, 0, stream, args, nullptr));
+      HIPCHK(va_reduce_cols_async(part, nblocks, 4, (uint64_t *)&ctr->tris_in, stream));
+      HIPCHK(va_fill_u32_async(cnt + tri_total, 1, 0u, stream));
+      HIPCHK(va_scan_u32((uint32_t *)d_temp, cnt, offs, tri_total + 1, stream));
+    }
+    else
+      HIPCHK(va_fill_u32_async(offs, 1, 0u, stream));
     HIPCHK(hipEventRecord(ev[1], stream));
+    HIPCHK(va_fill_u32_async(d_start, tile_total, 0u, stream));
+    HIPCHK(va_fill_u32_async(d_end, tile_total, 0u, stream));
+    uint32_t *bins = (uint32_t *)d_vals, *sorted_keys = (uint32_t *)d_keys;
+    if(tri_total)
+    {
+      uint32_t *keys = (uint32_t *)d_keys, *vals = (uint32_t *)d_vals;
+      void *eargs[] = {&rec, &offs, &tri_total, &cap, &dd, &df, (void *)&ts, &keys, &vals};
+      HIPCHK(hipModuleLaunchKernel(mod->emit, (tri_total + 255) / 256, 1, 1, 256, 1, 1, 0, stream, eargs, nullptr));
+      int in_alt = 0;
+      HIPCHK(va_sort_pairs((uint32_t *)d_stemp, keys, vals, (uint32_t *)d_keys2, (uint32_t *)d_vals2, total_dev, cap,
+                           BitsFor(tile_total), &in_alt, stream));
+      if(in_alt)
+      {
+        sorted_keys = (uint32_t *)d_keys2;
+        bins = (uint32_t *)d_vals2;
+      }
+      uint32_t *st_ = (uint32_t *)d_start, *en = (uint32_t *)d_end;
+      void *rargs[] = {&sorted_keys, &total_dev, &cap, &st_, &en};
+      HIPCHK(hipModuleLaunchKernel(mod->ranges, (cap + 255) / 256, 1, 1, 256, 1, 1, 0, stream, rargs, nullptr));
+    }
+    HIPCHK(hipEventRecord(ev[2], stream));
+    {
+      uint32_t *st_ = (uint32_t *)d_start, *en = (uint32_t *)d_end;
+      uint32_t use_fused = fused_clear ? 1u : 0u;
+      void *args[] = {&dd, &df, (void *)&nframes, &rec, &bins, &st_, &en, &tile_total, &use_fused, &part, &total_dev, &cap};
+      hipFunction_t k = mod->raster[ts - 3];
+      uint32_t threads = (T / 8) * (T / 8) * 64;
+      HIPCHK(hipModuleLaunchKernel(k, tile_total, 1, 1, threads, 1, 1, 0, stream, args, nullptr));
+      HIPCHK(hipEventRecord(ev[3], stream));
+      HIPCHK(va_reduce_cols_async(part, tile_total, 4, (uint64_t *)&ctr->pixels_written, stream));
+    }
+    HIPCHK(hipMemcpyAsync(h_pinned + 64, d_ctr, sizeof(VaCounters), hipMemcpyDeviceToHost, stream));
+    HIPCHK(hipMemcpyAsync(h_pinned, total_dev, 4, hipMemcpyDeviceToHost, stream));
     HIPCHK(hipStreamSynchronize(stream));
     total_pairs = h_pinned[0];
-  }
-  HIPCHK(va_fill_u32_async(d_start, tile_total, 0u, stream));
-  HIPCHK(va_fill_u32_async(d_end, tile_total, 0u, stream));
-  if(total_pairs)
-  {
-    if(!Grow(&d_keys, c_keys, 4ull * total_pairs) || !Grow(&d_vals, c_vals, 4ull * total_pairs) ||
-       !Grow(&d_keys2, c_keys2, 4ull * total_pairs) || !Grow(&d_vals2, c_vals2, 4ull * total_pairs))
+    if(total_pairs <= pair_cap)
+      break;
+    if(attempt > 0)
+    {
+      LogError("bin capacity did not converge (%u pairs)", total_pairs);
       return VK_ERROR_OUT_OF_DEVICE_MEMORY;
-    uint32_t *rec = (uint32_t *)d_rec, *offs = (uint32_t *)d_offs, *keys = (uint32_t *)d_keys, *vals = (uint32_t *)d_vals;
-    VaDraw *dd = (VaDraw *)d_draws;
-    VaFrame *df = (VaFrame *)d_frames;
-    void *args[] = {&rec, &offs, &tri_total, &total_pairs, &dd, &df, (void *)&ts, &keys, &vals};
-    HIPCHK(hipModuleLaunchKernel(mod->emit, (total_pairs + 255) / 256, 1, 1, 256, 1, 1, 0, stream, args, nullptr));
-    int bits = BitsFor(tile_total);
-    size_t need = 0;
-    HIPCHK(va_sort_pairs(nullptr, &need, keys, (uint32_t *)d_keys2, vals, (uint32_t *)d_vals2, total_pairs, bits, stream));
-    if(!Grow(&d_temp, c_temp, need))
-      return VK_ERROR_OUT_OF_DEVICE_MEMORY;
-    size_t have = c_temp;
-    HIPCHK(va_sort_pairs(d_temp, &have, keys, (uint32_t *)d_keys2, vals, (uint32_t *)d_vals2, total_pairs, bits, stream));
-    uint32_t *sk = (uint32_t *)d_keys2, *st_ = (uint32_t *)d_start, *en = (uint32_t *)d_end;
-    void *rargs[] = {&sk, &total_pairs, &st_, &en};
-    HIPCHK(hipModuleLaunchKernel(mod->ranges, (total_pairs + 255) / 256, 1, 1, 256, 1, 1, 0, stream, rargs, nullptr));
+    }
+    pair_cap = total_pairs + total_pairs / 4 + 4096;
   }
-  {
-    HIPCHK(hipEventRecord(ev[2], stream));
-    VaDraw *dd = (VaDraw *)d_draws;
-    VaFrame *df = (VaFrame *)d_frames;
-    uint32_t *rec = (uint32_t *)d_rec, *bins = (uint32_t *)d_vals2, *st_ = (uint32_t *)d_start, *en = (uint32_t *)d_end;
-    VaCounters *ctr = (VaCounters *)d_ctr;
-    uint64_t *part = (uint64_t *)d_part;
-    uint32_t use_fused = fused_clear ? 1u : 0u;
-    void *args[] = {&dd, &df, (void *)&nframes, &rec, &bins, &st_, &en, &tile_total, &use_fused, &part};
-    hipFunction_t k = mod->raster[ts - 3];
-    uint32_t threads = (T / 8) * (T / 8) * 64;
-    HIPCHK(hipModuleLaunchKernel(k, tile_total, 1, 1, threads, 1, 1, 0, stream, args, nullptr));
-    HIPCHK(hipEventRecord(ev[3], stream));
-    HIPCHK(va_reduce_cols_async(part, tile_total, 4, (uint64_t *)&ctr->pixels_written, stream));
-  }
-  HIPCHK(hipMemcpyAsync(h_pinned + 64, d_ctr, sizeof(VaCounters), hipMemcpyDeviceToHost, stream));
-  HIPCHK(hipStreamSynchronize(stream));
   VaCounters c;
   memcpy(&c, h_pinned + 64, sizeof(c));
   c.draws = ndraws;

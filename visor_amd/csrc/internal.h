@@ -331,9 +331,18 @@ enum class ExecMode
   Batched = 1,  // every draw of a render pass in one setup/bin/raster sequence, fused clears
 };
 
+// Launch shape of the raster stage. tile_shift selects the kernel at launch; chunk (bin
+// entries staged in LDS per round, 16..64) and small (largest bbox-in-block overlap tested
+// triangle-parallel) are compile-time parameters of the pipeline module.
 struct TileConfig
 {
-  uint32_t tile_shift = 4;  // 3, 4, 5 -> 8, 16, 32 pixel tiles
+  uint32_t tile_shift = 3;  // 3, 4, 5 -> 8, 16, 32 pixel tiles
+  uint32_t chunk = 64;
+  uint32_t small = 16;
+  bool operator==(const TileConfig &o) const
+  {
+    return tile_shift == o.tile_shift && chunk == o.chunk && small == o.small;
+  }
 };
 
 class Device
@@ -349,7 +358,7 @@ public:
   std::shared_ptr<ShaderPair> MakePair(VkShaderModule_T *vs, const char *vs_entry, VkShaderModule_T *fs,
                                        const char *fs_entry, std::string &err);
   std::shared_ptr<PipelineModule> GetModule(const std::vector<std::shared_ptr<ShaderPair>> &pairs,
-                                            std::string &err);
+                                            std::string &err, const TileConfig &cfg = TileConfig());
   std::string cache_dir;
 
 private:
@@ -392,6 +401,9 @@ private:
   size_t c_blocks = 0;
   void *d_part = nullptr;  // per-workgroup counter partials
   size_t c_part = 0;
+  void *d_stemp = nullptr;  // radix sort histograms
+  size_t c_stemp = 0;
+  uint32_t pair_cap = 0;    // (tile, primitive) pair capacity of the bin buffers
   size_t c_frames = 0, c_draws = 0, c_rec = 0, c_count = 0, c_offs = 0, c_keys = 0, c_vals = 0, c_keys2 = 0,
          c_vals2 = 0, c_start = 0, c_end = 0, c_ctr = 0, c_temp = 0;
   uint32_t *h_pinned = nullptr;  // small pinned staging for readbacks

@@ -8,8 +8,9 @@
 // Stages (reference rasterizer.cpp:363-696 in GPU form):
 //   va_setup   one thread per primitive: index fetch, 3 VS invocations, ToWindow, area / facing
 //              / cull, clamped bbox, 64..128-byte triangle record, per-primitive tile count
-//   va_emit    one thread per (tile, primitive) pair, keys = global tile id, values = primitive
-//   (stable radix sort by tile, in the host runtime)  -> bins keep API primitive order
+//   va_emit    (tile, primitive) pairs, keys = global tile id, values = primitive, expanded per
+//              group of 256 primitives with an LDS search
+//   (stable radix sort by tile, runtime_kernels.hip)  -> bins keep API primitive order
 //   va_ranges  per-tile [start, end) into the sorted values
 //   va_raster  one workgroup per tile; each wave owns an 8x8 pixel block with colour and depth in
 //              registers, walks the tile's bin in order: integer edge functions + ballot,
@@ -197,37 +198,50 @@ extern "C" __global__ __launch_bounds__(256) void va_setup(
   va_block_partials(part + (u64)blockIdx.x * 4u, n_in, n_out, n_blocks, n_tested);
 }
 
-// One thread per (tile, primitive) pair. offsets = exclusive scan of tile counts (ntris + 1).
+// Pair generation, load-balanced: workgroup b expands primitives [256 b, 256 b + 256). offsets
+// is the exclusive scan of the per-primitive tile counts (total_tris + 1 entries, the last one
+// the pair total); each thread takes pairs of the group by stride and finds its primitive by a
+// binary search over the group's 257 offsets in LDS. Pairs at or beyond cap are not written
+// (the host sees the total and grows the buffers).
 extern "C" __global__ __launch_bounds__(256) void va_emit(
-    const u32 *__restrict__ rec, const u32 *__restrict__ offsets, u32 total_tris, u32 total_pairs,
+    const u32 *__restrict__ rec, const u32 *__restrict__ offsets, u32 total_tris, u32 cap,
     const VaDraw *__restrict__ draws, const VaFrame *__restrict__ frames, u32 tile_shift,
     u32 *__restrict__ keys, u32 *__restrict__ vals)
 {
-  const u32 p = blockIdx.x * blockDim.x + threadIdx.x;
-  if(p >= total_pairs) return;
-  u32 lo = 0, hi = total_tris;
-  while(hi - lo > 1u)
+  __shared__ u32 s_off[257];
+  const u32 t0 = blockIdx.x * 256u;
+  s_off[threadIdx.x] = offsets[min(t0 + threadIdx.x, total_tris)];
+  if(threadIdx.x == 0u)
+    s_off[256] = offsets[min(t0 + 256u, total_tris)];
+  __syncthreads();
+  const u32 pend = min(s_off[256], cap);
+  for(u32 p = s_off[0] + threadIdx.x; p < pend; p += 256u)
   {
-    u32 mid = (lo + hi) >> 1;
-    if(offsets[mid] <= p) lo = mid; else hi = mid;
+    // last i with s_off[i] <= p (the steps sum to 255, so i + step stays below 256)
+    u32 i = 0;
+#pragma unroll
+    for(u32 step = 128u; step > 0u; step >>= 1)
+      if(s_off[i + step] <= p)
+        i += step;
+    const u32 t = t0 + i;
+    const u32 k = p - s_off[i];
+    const u32 *r = rec + (u64)t * VA_REC_STRIDE;
+    const u32 bmin = r[VA_R_BBMIN], bmax = r[VA_R_BBMAX];
+    const u32 tx0 = (bmin & 0xffffu) >> tile_shift, tx1 = ((bmax & 0xffffu) - 1u) >> tile_shift;
+    const u32 ty0 = (bmin >> 16) >> tile_shift;
+    const u32 nx = tx1 - tx0 + 1u;
+    const VaFrame &fr = frames[draws[r[VA_R_DRAW]].frame];
+    const u32 tx = tx0 + k % nx, ty = ty0 + k / nx;
+    keys[p] = fr.tile_base + ty * fr.tiles_x + tx;
+    vals[p] = t;
   }
-  const u32 t = lo;
-  const u32 k = p - offsets[t];
-  const u32 *r = rec + (u64)t * VA_REC_STRIDE;
-  const u32 bmin = r[VA_R_BBMIN], bmax = r[VA_R_BBMAX];
-  const u32 tx0 = (bmin & 0xffffu) >> tile_shift, tx1 = ((bmax & 0xffffu) - 1u) >> tile_shift;
-  const u32 ty0 = (bmin >> 16) >> tile_shift;
-  const u32 nx = tx1 - tx0 + 1u;
-  const VaFrame &fr = frames[draws[r[VA_R_DRAW]].frame];
-  const u32 tx = tx0 + k % nx, ty = ty0 + k / nx;
-  keys[p] = fr.tile_base + ty * fr.tiles_x + tx;
-  vals[p] = t;
 }
 
-extern "C" __global__ __launch_bounds__(256) void va_ranges(const u32 *__restrict__ keys, u32 n,
-                                                            u32 *__restrict__ start,
-                                                            u32 *__restrict__ end)
+// Bin ranges from the sorted keys: [start, end) per tile. n = min(*n_dev, cap).
+extern "C" __global__ __launch_bounds__(256) void va_ranges(const u32 *__restrict__ keys, const u32 *__restrict__ n_dev,
+                                                            u32 cap, u32 *__restrict__ start, u32 *__restrict__ end)
 {
+  const u32 n = min(*n_dev, cap);
   const u32 i = blockIdx.x * blockDim.x + threadIdx.x;
   if(i >= n) return;
   const u32 k = keys[i];
@@ -369,7 +383,9 @@ VA_INLINE void va_wave_sync()
 #ifndef VA_CHUNK
 #define VA_CHUNK 64u       // bin entries staged in LDS per round (at most 64: one bit each)
 #endif
+#ifndef VA_SMALL
 #define VA_SMALL 16        // bbox-in-block overlaps up to this many pixels are tested per triangle
+#endif
 static_assert(VA_CHUNK <= 64u, "record sets are 64-bit masks");
 
 // Transposes the 64x64 bit matrix held one row per lane (bit j of lane i = element (i, j)):
@@ -722,14 +738,17 @@ VA_INLINE void va_raster_tile(const VaDraw *__restrict__ draws, const VaFrame *_
   va_block_partials(row, n_cov, n_pass, n_blend, n_dw);
 }
 
+// pair_total / pair_cap: when the pairs overflowed the bins the launch writes nothing, so the
+// host can grow the buffers and run the pass again on an untouched target.
 #define VA_RASTER_KERNEL(T)                                                                                 \
   extern "C" __global__ __launch_bounds__((T / 8) * (T / 8) * 64) void va_raster_##T(                      \
       const VaDraw *__restrict__ draws, const VaFrame *__restrict__ frames, u32 nframes,                  \
       const u32 *__restrict__ rec, const u32 *__restrict__ bins, const u32 *__restrict__ bin_start,        \
-      const u32 *__restrict__ bin_end, u32 ntiles, u32 use_fused_clear, u64 *__restrict__ part)            \
+      const u32 *__restrict__ bin_end, u32 ntiles, u32 use_fused_clear, u64 *__restrict__ part,            \
+      const u32 *__restrict__ pair_total, u32 pair_cap)                                                     \
   {                                                                                                         \
     const u32 tile = blockIdx.x;                                                                            \
-    if(tile < ntiles)                                                                                       \
+    if(tile < ntiles && *pair_total <= pair_cap)                                                            \
       va_raster_tile<T>(draws, frames, nframes, rec, bins, bin_start, bin_end, use_fused_clear, part, tile); \
   }
 

@@ -8,11 +8,15 @@
 extern "C" {
 #endif
 
-hipError_t va_scan_u32(void *temp, size_t *temp_bytes, const uint32_t *in, uint32_t *out, uint32_t n,
-                       hipStream_t stream);
-hipError_t va_sort_pairs(void *temp, size_t *temp_bytes, const uint32_t *keys_in, uint32_t *keys_out,
-                         const uint32_t *vals_in, uint32_t *vals_out, uint32_t n, int end_bit,
-                         hipStream_t stream);
+/* Exclusive scan of n values (out may alias in); temp holds va_scan_temp_words(n) words. */
+size_t va_scan_temp_words(uint32_t n);
+hipError_t va_scan_u32(uint32_t *temp, const uint32_t *in, uint32_t *out, uint32_t n, hipStream_t stream);
+/* Stable LSD radix sort of the first min(*n_dev, cap) (key, value) pairs over the low `bits` key
+ * bits, ping-ponging between (keys, vals) and (keys_alt, vals_alt); *result_in_alt says which
+ * pair of arrays holds the result. temp holds va_sort_temp_words(cap) words. */
+size_t va_sort_temp_words(uint32_t cap);
+hipError_t va_sort_pairs(uint32_t *temp, uint32_t *keys, uint32_t *vals, uint32_t *keys_alt, uint32_t *vals_alt,
+                         const uint32_t *n_dev, uint32_t cap, int bits, int *result_in_alt, hipStream_t stream);
 hipError_t va_fill_u32_async(void *p, uint64_t count, uint32_t value, hipStream_t stream);
 hipError_t va_fill_u64_async(void *p, uint64_t count, uint32_t lo, uint32_t hi, hipStream_t stream);
 

@@ -1,27 +1,286 @@
 // runtime_kernels.hip - pipeline-independent device work, compiled ahead of time for gfx950:
 // the exclusive scan of per-primitive tile counts, the stable radix sort of (tile, primitive)
-// pairs that turns them into API-ordered bins, and render-target clears.
-#include <cstring>
-
+// pairs that turns them into API-ordered bins, fills and counter reductions. Every launch here
+// is a plain kernel (no library calls, no memset nodes), so a frame's sequence can be captured
+// into a HIP graph.
 #include <hip/hip_runtime.h>
-#include <rocprim/device/device_radix_sort.hpp>
-#include <rocprim/device/device_scan.hpp>
 
 #include "runtime_kernels.h"
 
-extern "C" hipError_t va_scan_u32(void *temp, size_t *temp_bytes, const uint32_t *in, uint32_t *out,
-                                  uint32_t n, hipStream_t stream)
+typedef uint32_t u32;
+typedef uint64_t u64;
+
+// ---------------------------------------------------------------------------------------------
+// Wave and block scans (64-lane wavefronts)
+
+__device__ __forceinline__ u32 rk_wave_inclusive(u32 v, u32 lane)
 {
-  return rocprim::exclusive_scan(temp, *temp_bytes, in, out, 0u, (size_t)n, rocprim::plus<uint32_t>(), stream);
+#pragma unroll
+  for(u32 o = 1u; o < 64u; o <<= 1)
+  {
+    const u32 t = (u32)__shfl_up((int)v, o);
+    if(lane >= o)
+      v += t;
+  }
+  return v;
 }
 
-// LSD radix sort is stable: pairs emitted in primitive order stay in primitive order per tile.
-extern "C" hipError_t va_sort_pairs(void *temp, size_t *temp_bytes, const uint32_t *keys_in,
-                                    uint32_t *keys_out, const uint32_t *vals_in, uint32_t *vals_out,
-                                    uint32_t n, int end_bit, hipStream_t stream)
+// Exclusive scan over the block (blockDim.x = 64 * NW threads); s_w holds NW words.
+template <u32 NW>
+__device__ __forceinline__ u32 rk_block_exclusive(u32 v, u32 *s_w, u32 &total)
 {
-  return rocprim::radix_sort_pairs(temp, *temp_bytes, keys_in, keys_out, vals_in, vals_out, (size_t)n, 0,
-                                   end_bit, stream);
+  const u32 tid = threadIdx.x, lane = tid & 63u, w = tid >> 6;
+  const u32 incl = rk_wave_inclusive(v, lane);
+  if(lane == 63u)
+    s_w[w] = incl;
+  __syncthreads();
+  u32 off = 0, tot = 0;
+#pragma unroll
+  for(u32 k = 0; k < NW; k++)
+  {
+    const u32 x = s_w[k];
+    off += k < w ? x : 0u;
+    tot += x;
+  }
+  total = tot;
+  __syncthreads();
+  return off + incl - v;
+}
+
+__device__ __forceinline__ void rk_wave_sync()
+{
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// ---------------------------------------------------------------------------------------------
+// Exclusive scan of n u32 (reduce, scan of block sums, down-sweep). 2048 items per block.
+
+#define RK_SCAN_T 256u
+#define RK_SCAN_I 8u
+#define RK_SCAN_B (RK_SCAN_T * RK_SCAN_I)
+
+__global__ __launch_bounds__(256) void va_scan_reduce_kernel(const u32 *__restrict__ in, u32 n, u32 *__restrict__ bsum)
+{
+  __shared__ u32 s_w[4];
+  const u32 base = blockIdx.x * RK_SCAN_B;
+  u32 t = 0;
+#pragma unroll
+  for(u32 i = 0; i < RK_SCAN_I; i++)
+  {
+    const u32 idx = base + i * RK_SCAN_T + threadIdx.x;
+    t += idx < n ? in[idx] : 0u;
+  }
+  u32 total;
+  rk_block_exclusive<4>(t, s_w, total);
+  if(threadIdx.x == 0)
+    bsum[blockIdx.x] = total;
+}
+
+// One workgroup of 1024 threads scans the block sums in place, 1024 at a time.
+__global__ __launch_bounds__(1024) void va_scan_top_kernel(u32 *__restrict__ bsum, u32 nb)
+{
+  __shared__ u32 s_w[16];
+  u32 carry = 0;
+  for(u32 c = 0; c < nb; c += 1024u)
+  {
+    const u32 idx = c + threadIdx.x;
+    const u32 v = idx < nb ? bsum[idx] : 0u;
+    u32 total;
+    const u32 ex = rk_block_exclusive<16>(v, s_w, total);
+    if(idx < nb)
+      bsum[idx] = carry + ex;
+    carry += total;
+  }
+}
+
+// out may alias in: every block reads its whole range into LDS before writing it.
+__global__ __launch_bounds__(256) void va_scan_down_kernel(const u32 *in, u32 n, const u32 *__restrict__ bsum, u32 *out)
+{
+  __shared__ u32 s_v[RK_SCAN_B];
+  __shared__ u32 s_w[4];
+  const u32 base = blockIdx.x * RK_SCAN_B;
+#pragma unroll
+  for(u32 i = 0; i < RK_SCAN_I; i++)
+  {
+    const u32 idx = base + i * RK_SCAN_T + threadIdx.x;
+    s_v[i * RK_SCAN_T + threadIdx.x] = idx < n ? in[idx] : 0u;
+  }
+  __syncthreads();
+  u32 v[RK_SCAN_I], t = 0;
+#pragma unroll
+  for(u32 i = 0; i < RK_SCAN_I; i++)
+  {
+    v[i] = s_v[threadIdx.x * RK_SCAN_I + i];
+    t += v[i];
+  }
+  u32 total;
+  u32 run = bsum[blockIdx.x] + rk_block_exclusive<4>(t, s_w, total);
+#pragma unroll
+  for(u32 i = 0; i < RK_SCAN_I; i++)
+  {
+    s_v[threadIdx.x * RK_SCAN_I + i] = run;
+    run += v[i];
+  }
+  __syncthreads();
+#pragma unroll
+  for(u32 i = 0; i < RK_SCAN_I; i++)
+  {
+    const u32 idx = base + i * RK_SCAN_T + threadIdx.x;
+    if(idx < n)
+      out[idx] = s_v[i * RK_SCAN_T + threadIdx.x];
+  }
+}
+
+static u32 scan_blocks(u32 n) { return (n + RK_SCAN_B - 1) / RK_SCAN_B; }
+
+extern "C" size_t va_scan_temp_words(uint32_t n) { return (size_t)scan_blocks(n) + 64; }
+
+extern "C" hipError_t va_scan_u32(uint32_t *temp, const uint32_t *in, uint32_t *out, uint32_t n, hipStream_t stream)
+{
+  if(n == 0)
+    return hipSuccess;
+  const u32 nb = scan_blocks(n);
+  hipLaunchKernelGGL(va_scan_reduce_kernel, dim3(nb), dim3(RK_SCAN_T), 0, stream, in, n, temp);
+  hipLaunchKernelGGL(va_scan_top_kernel, dim3(1), dim3(1024), 0, stream, temp, nb);
+  hipLaunchKernelGGL(va_scan_down_kernel, dim3(nb), dim3(RK_SCAN_T), 0, stream, in, n, (const u32 *)temp, out);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------------------------
+// Stable LSD radix sort of (key, value) pairs, 8-bit digits, 4096 pairs per workgroup:
+//   histogram per (digit, workgroup) -> exclusive scan in digit-major order -> stable scatter.
+// The pair count is read on the device (min(*n_dev, cap)), so the launches are sized by the
+// capacity and need no host readback.
+
+#define RK_SORT_T 256u
+#define RK_SORT_I 16u
+#define RK_SORT_B (RK_SORT_T * RK_SORT_I)
+
+__global__ __launch_bounds__(256) void va_sort_hist_kernel(const u32 *__restrict__ keys, const u32 *__restrict__ n_dev,
+                                                           u32 cap, u32 shift, u32 *__restrict__ hist, u32 nbs)
+{
+  __shared__ u32 s_h[256];
+  s_h[threadIdx.x] = 0u;
+  __syncthreads();
+  const u32 n = min(*n_dev, cap);
+  const u32 base = blockIdx.x * RK_SORT_B;
+#pragma unroll 4
+  for(u32 i = 0; i < RK_SORT_I; i++)
+  {
+    const u32 idx = base + i * RK_SORT_T + threadIdx.x;
+    if(idx < n)
+      atomicAdd(&s_h[(keys[idx] >> shift) & 255u], 1u);
+  }
+  __syncthreads();
+  hist[threadIdx.x * nbs + blockIdx.x] = s_h[threadIdx.x];
+}
+
+// Wave w of the workgroup ranks pairs [base + 1024 w, base + 1024 (w + 1)) in order, 64 at a time:
+// lanes with equal digits find each other with eight ballots, and per-wave digit counters in LDS
+// carry the rank across steps. Waves are then ordered by a per-digit prefix over the waves.
+__global__ __launch_bounds__(256) void va_sort_scatter_kernel(const u32 *__restrict__ kin, const u32 *__restrict__ vin,
+                                                              u32 *__restrict__ kout, u32 *__restrict__ vout,
+                                                              const u32 *__restrict__ n_dev, u32 cap, u32 shift,
+                                                              const u32 *__restrict__ hist, u32 nbs)
+{
+  __shared__ u32 s_cnt[4][256];
+  __shared__ u32 s_goff[256];
+  const u32 tid = threadIdx.x, lane = tid & 63u, w = tid >> 6;
+  const u32 n = min(*n_dev, cap);
+  const u32 base = blockIdx.x * RK_SORT_B;
+  if(base >= n)
+    return;
+  s_goff[tid] = hist[tid * nbs + blockIdx.x];
+#pragma unroll
+  for(u32 k = 0; k < 4u; k++)
+    s_cnt[k][tid] = 0u;
+  __syncthreads();
+  const u64 below = lane == 0u ? 0ull : (~0ull >> (64u - lane));
+  u32 key[RK_SORT_I], val[RK_SORT_I], rank[RK_SORT_I];
+#pragma unroll
+  for(u32 j = 0; j < RK_SORT_I; j++)
+  {
+    const u32 idx = base + w * (RK_SORT_I * 64u) + j * 64u + lane;
+    const bool valid = idx < n;
+    const u32 k = valid ? kin[idx] : 0u;
+    const u32 v = valid ? vin[idx] : 0u;
+    const u32 d = (k >> shift) & 255u;
+    u64 peers = __ballot(valid);
+#pragma unroll
+    for(u32 b = 0; b < 8u; b++)
+    {
+      const u64 m = __ballot((d >> b) & 1u);
+      peers &= ((d >> b) & 1u) ? m : ~m;
+    }
+    const u32 r = (u32)__popcll(peers & below);
+    const u32 cur = valid ? s_cnt[w][d] : 0u;
+    rk_wave_sync();
+    if(valid && r == 0u)
+      s_cnt[w][d] = cur + (u32)__popcll(peers);
+    rk_wave_sync();
+    key[j] = k;
+    val[j] = v;
+    rank[j] = valid ? cur + r : 0xffffffffu;
+  }
+  __syncthreads();
+  {
+    u32 run = 0;
+#pragma unroll
+    for(u32 k = 0; k < 4u; k++)
+    {
+      const u32 c = s_cnt[k][tid];
+      s_cnt[k][tid] = run;
+      run += c;
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for(u32 j = 0; j < RK_SORT_I; j++)
+  {
+    if(rank[j] != 0xffffffffu)
+    {
+      const u32 d = (key[j] >> shift) & 255u;
+      const u32 pos = s_goff[d] + s_cnt[w][d] + rank[j];
+      kout[pos] = key[j];
+      vout[pos] = val[j];
+    }
+  }
+}
+
+static u32 sort_blocks(u32 cap) { return (cap + RK_SORT_B - 1) / RK_SORT_B; }
+
+extern "C" size_t va_sort_temp_words(uint32_t cap)
+{
+  const u32 nh = 256u * sort_blocks(cap);
+  return (size_t)nh + va_scan_temp_words(nh);
+}
+
+extern "C" hipError_t va_sort_pairs(uint32_t *temp, uint32_t *keys, uint32_t *vals, uint32_t *keys_alt,
+                                    uint32_t *vals_alt, const uint32_t *n_dev, uint32_t cap, int bits,
+                                    int *result_in_alt, hipStream_t stream)
+{
+  *result_in_alt = 0;
+  if(cap == 0)
+    return hipSuccess;
+  const u32 nbs = sort_blocks(cap), nh = 256u * nbs;
+  u32 *hist = temp, *scan_tmp = temp + nh;
+  u32 *ki = keys, *vi = vals, *ko = keys_alt, *vo = vals_alt;
+  for(int shift = 0; shift < bits; shift += 8)
+  {
+    hipLaunchKernelGGL(va_sort_hist_kernel, dim3(nbs), dim3(RK_SORT_T), 0, stream, (const u32 *)ki, n_dev, cap,
+                       (u32)shift, hist, nbs);
+    hipError_t e = va_scan_u32(scan_tmp, hist, hist, nh, stream);
+    if(e != hipSuccess)
+      return e;
+    hipLaunchKernelGGL(va_sort_scatter_kernel, dim3(nbs), dim3(RK_SORT_T), 0, stream, (const u32 *)ki,
+                       (const u32 *)vi, ko, vo, n_dev, cap, (u32)shift, (const u32 *)hist, nbs);
+    u32 *t = ki; ki = ko; ko = t;
+    t = vi; vi = vo; vo = t;
+    *result_in_alt ^= 1;
+  }
+  return hipGetLastError();
 }
 
 __global__ __launch_bounds__(256) void va_fill32_kernel(uint32_t *p, uint64_t n, uint32_t v)
