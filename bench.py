@@ -51,16 +51,52 @@ def profiled_traffic(tile):
     return None
 
 
+def serve(ctx, args):
+    """64 clients, Poisson arrivals at each rate in turn: served fragments/s and submit -> present
+    latency (the client's colour target back in host memory)."""
+    from visor_amd import serving
+    t0 = time.perf_counter()
+    world = serving.ServingWorld(ctx, nclients=args.serve_clients, seed=4)
+    srv = serving.Server(ctx, hbm_budget_bytes=16 << 30, slots=2, max_batch=64)
+    for c in world.clients:  # warm: every client once
+        srv.wait(srv.submit(c.cb))
+    print("[bench] serving world: %d clients, %d-%d triangles per request, built in %.1f s" % (
+        len(world.clients), min(c.triangles for c in world.clients), max(c.triangles for c in world.clients),
+        time.perf_counter() - t0), file=sys.stderr, flush=True)
+    out = {"clients": len(world.clients), "slots": 2, "hbm_budget_gib": 16, "seconds_per_rate": args.serve_seconds,
+           "workload": "targets 256^2/512^2/1024^2/1920x1080 BGRA8+D32, 1-32 indexed draws of resident UV-sphere "
+                       "meshes (log-uniform total triangles, 1k-500k), Lambert x mipmapped texture 256^2-2048^2, "
+                       "bilinear clamp, back-face cull", "rates": []}
+    for k, rate in enumerate(float(x) for x in args.serve_rates.split(",")):
+        st, wall = serving.run_clients(world, srv, rate, args.serve_seconds, seed=k + 1)
+        row = {"offered_rps": rate, "completed": st["completed"], "achieved_rps": round(st["completed"] / wall, 1),
+               "gfragments_s": round(st["fragments"] / wall / 1e9, 4), "p50_ms": round(st["p50_ms"], 3),
+               "p99_ms": round(st["p99_ms"], 3), "mean_batch": round(st["mean_batch"], 2),
+               "gpu_busy_frac": round(st["busy_ms"] / (wall * 1e3), 3)}
+        out["rates"].append(row)
+        print("[bench] served %s" % json.dumps(row), file=sys.stderr, flush=True)
+    srv.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=12, help="untimed frames; they also let the autotuner try "
+                    "its 5 launch shapes twice")
     ap.add_argument("--plain-steps", type=int, default=3)
-    ap.add_argument("--tile", type=int, default=8)
+    ap.add_argument("--tile", type=int, default=0, help="8, 16 or 32 fixes the tile size; 0 = autotuned")
     ap.add_argument("--cpu-draws", type=int, default=128, help="draws in the CPU-baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-graphs", action="store_true", help="launch the batched pass kernel by kernel")
+    ap.add_argument("--no-serve", action="store_true", help="skip the 64-client serving run")
+    ap.add_argument("--serve-rates", type=str, default="1000,4000,1000000",
+                    help="aggregate Poisson arrival rates (requests/s) of the serving run; a rate far above "
+                         "capacity makes every client resubmit as soon as its frame is done")
+    ap.add_argument("--serve-seconds", type=float, default=3.0)
+    ap.add_argument("--serve-clients", type=int, default=64)
     ap.add_argument("--calib-copy", action="store_true",
                     help="run only a 1 GiB device copy (2 GiB of HBM traffic) to calibrate FETCH_SIZE/WRITE_SIZE")
     args = ap.parse_args()
@@ -90,7 +126,7 @@ def main():
     scene = scenes.config3()
     ctx = render.Context()
     r = render.SceneRenderer(ctx, scene, host_visible_targets=False)
-    ctx.set_mode(batched=True, tile=args.tile)
+    ctx.set_mode(batched=True, tile=args.tile, graphs=not args.no_graphs)
     cb = r.record()
     r.cb = cb
 
@@ -99,6 +135,7 @@ def main():
     for _ in range(args.warmup):
         r.render()
     stats_one = ctx.stats(0)
+    tile = stats_one["tile_size"]
     print("[bench] frame stats: %s" % json.dumps(stats_one), file=sys.stderr, flush=True)
 
     # timed batched steps
@@ -121,12 +158,12 @@ def main():
     b_raster = raster_bytes(stats_one, scene.width, scene.height, 4, 32)
     achieved = b_raster / (raster_ms * 1e-3) / 1e9
     roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": profiled_traffic(args.tile)}
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": profiled_traffic(tile)}
 
     # plain per-draw path
     plain = None
     if not args.no_plain:
-        ctx.set_mode(batched=False, tile=args.tile)
+        ctx.set_mode(batched=False)
         r.render()
         ctx.icd.vh_reset_stats(ctx.queue)
         t0 = time.perf_counter()
@@ -135,7 +172,7 @@ def main():
         pe = time.perf_counter() - t0
         plain = frags_per_frame * args.plain_steps / pe / 1e9
         print("[bench] plain %.3f ms/frame" % (pe * 1e3 / args.plain_steps), file=sys.stderr, flush=True)
-        ctx.set_mode(batched=True, tile=args.tile)
+        ctx.set_mode(batched=True)
 
     cpu = None
     if not args.no_cpu:
@@ -148,6 +185,10 @@ def main():
                          "7 workers + work-stealing submitter over 32x32 blocks" % (
                              k, res["counters"]["pixels_written"], res["seconds"])}
         print("[bench] cpu baseline %.4f Gfrag/s" % cpu["value"], file=sys.stderr, flush=True)
+
+    served = None
+    if not args.no_serve:
+        served = serve(ctx, args)
 
     lat_ms = sorted(x * 1e3 for x in lat)
     out = {
@@ -165,7 +206,8 @@ def main():
         "data": "synthetic",
         "config": {"workload": "config3: 1920x1080 BGRA8+D32, 256 draws, %d triangles, trilinear repeat, "
                                "half alpha-blended" % scene.triangles,
-                   "mode": "batched", "tile": args.tile, "fragments_per_frame": frags_per_frame,
+                   "mode": "batched", "tile": tile, "chunk": stats_one["chunk"],
+                   "launch_shape": "autotuned" if args.tile == 0 else "fixed", "fragments_per_frame": frags_per_frame,
                    "shaded_per_frame": stats_one["depth_passed"]},
         "plain_per_draw_gfrag_s": round(plain, 4) if plain else None,
         "speedup_vs_plain": round(value / plain, 2) if plain else None,
@@ -173,8 +215,10 @@ def main():
                              "p99": round(lat_ms[min(len(lat_ms) - 1, int(0.99 * len(lat_ms)))], 3)},
         "kernel_ms": {"raster": round(raster_ms, 4), "setup": round(acc["setup_ms"] / max(args.steps, 1), 4),
                       "bin": round(acc["bin_ms"] / max(args.steps, 1), 4)},
+        "graph_replays": acc["graph_launches"],
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "served": served,
     }
     print(json.dumps(out))
 

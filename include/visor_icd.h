@@ -55,6 +55,9 @@ typedef struct VhStats {
   uint64_t depth_passed;    /* shaded fragments (rasterizer/depth/passed)                  */
   uint64_t shaded_blend;    /* shaded fragments of blending draws                          */
   uint64_t shaded_dwrite;   /* shaded fragments that wrote depth                           */
+  uint64_t graph_launches;  /* batched passes replayed from a captured HIP graph             */
+  uint64_t tile_size;       /* launch shape of the last pass: tile side (8, 16, 32) ...      */
+  uint64_t chunk;           /* ... and bin entries staged in LDS per round                  */
 } VhStats;
 
 /* 1 when a HIP device is usable. */
@@ -63,7 +66,12 @@ int vh_device_available(void);
 const char *vh_last_error(void);
 /* 0 = plain per-draw execution (host-synchronised per draw), 1 = batched (default). */
 int vh_set_exec_mode(VkQueue queue, int mode);
-/* Screen tile side for binning and rasterization: 8 (default), 16 or 32. */
+/* 1 (default) = a batched pass's device work is captured once into a HIP graph and replayed
+ * while its shape (sizes, buffers, pipelines) repeats; 0 = launched kernel by kernel. */
+int vh_set_graph_mode(VkQueue queue, int on);
+/* Screen tile side for binning and rasterization: 8, 16 or 32 fixes the launch shape; 0 (the
+ * default) lets the autotuner pick tile size and LDS chunk per pass shape, by timing each
+ * candidate on the pass's first repetitions (every candidate renders identical bytes). */
 int vh_set_tile_size(VkQueue queue, int tile);
 /* 0 = vkQueueSubmit returns after the GPU finished (the reference's behaviour), 1 = returns at
  * once; completion through fences / vkQueueWaitIdle. */
@@ -78,6 +86,38 @@ int vh_translate_spirv(const uint32_t *code, size_t nwords, int stage, char *out
  * the code object size or -1 with the error text. */
 int vh_build_pipeline_offline(const uint32_t *vs, size_t vs_words, const uint32_t *fs,
                               size_t fs_words, const char *arch, char *err_out, size_t err_size);
+
+/* ---- serving (no counterpart in the reference) -------------------------------------------
+ * Many clients hand recorded command buffers to one server. Pending requests are admitted in
+ * arrival order while the device bytes they touch (targets, textures, triangle records and
+ * bins) fit the HBM budget beside the requests already on the GPU; admitted requests' render
+ * passes, whatever their target sizes, share one setup / bin / raster launch sequence; up to
+ * `slots` batches run at once on separate streams. A request completes when its host-visible
+ * targets hold the result (submit -> present). Requests writing the same memory never share a
+ * batch or overlap, so every result equals the one the request gives alone. */
+typedef struct VhServer_T *VhServer;
+typedef struct VhServerConfig {
+  uint64_t hbm_budget_bytes;  /* admission limit (0: 16 GiB)                                */
+  uint32_t slots;             /* batches on the GPU at once, 1..4                           */
+  uint32_t max_batch;         /* requests merged into one launch sequence                   */
+} VhServerConfig;
+typedef struct VhServerStats {
+  uint64_t submitted, completed, batches;
+  uint64_t fragments;         /* covered fragments of completed requests                    */
+  uint64_t shaded;            /* shaded fragments of completed requests                     */
+  uint64_t max_inflight_bytes;
+  double busy_ms;             /* wall time with at least one batch executing               */
+  double mean_batch;          /* requests per batch                                         */
+  double p50_ms, p99_ms, mean_ms, max_ms;  /* submit -> present latency                     */
+} VhServerStats;
+VkResult vh_server_create(VkDevice device, const VhServerConfig *cfg, VhServer *out);
+void vh_server_destroy(VhServer server);
+/* Queues a request; returns at once with its ticket. */
+VkResult vh_server_submit(VhServer server, VkCommandBuffer cb, uint64_t *ticket);
+/* Waits for a request (timeout in ns, UINT64_MAX = forever); latency_ms may be NULL. */
+VkResult vh_server_wait(VhServer server, uint64_t ticket, uint64_t timeout_ns, double *latency_ms);
+int vh_server_stats(VhServer server, VhServerStats *out);
+int vh_server_reset_stats(VhServer server);
 
 #ifdef __cplusplus
 }

@@ -81,3 +81,43 @@ def test_config3_full_batched_equals_plain(ctx):
     assert np.array_equal(a["color"], b["color"])
     assert np.array_equal(a["depth"].view(np.uint32), b["depth"].view(np.uint32))
     assert a["stats"]["pixels_written"] == b["stats"]["pixels_written"]
+
+
+def test_graph_replays_match_eager(ctx):
+    """A batched pass captured into a HIP graph and replayed, with other (eager, per-draw) work
+    and host synchronisation between replays, gives the eager result on every replay."""
+    s = scenes.config3(ndraws=24, width=320, height=240, tex_size=64, max_tris=2048)
+    r = render.SceneRenderer(ctx, s)
+    o = render.SceneRenderer(ctx, scenes.config1(300, 96, 96, seed=7))
+    ctx.set_mode(batched=True, tile=8, graphs=False)
+    r.render()
+    ref_c, ref_d = r.read_color().copy(), r.read_depth().copy()
+    ctx.set_mode(batched=True, tile=8, graphs=True)
+    ctx.icd.vh_reset_stats(ctx.queue)
+    for i in range(4):
+        r.render()
+        assert np.array_equal(r.read_color(), ref_c), i
+        assert np.array_equal(r.read_depth().view(np.uint32), ref_d.view(np.uint32)), i
+        ctx.set_mode(batched=False)
+        o.render()
+        ctx.set_mode(batched=True)
+    assert ctx.stats(1)["graph_launches"] >= 4
+
+
+def test_autotuner_shapes_render_identical_bytes(ctx):
+    """With the autotuner on, the first repetitions of a pass run under each candidate launch
+    shape (tile size, waves per workgroup, LDS chunk); every repetition must give the same bytes,
+    and the tuner settles on one candidate."""
+    s = scenes.config3(ndraws=24, width=320, height=240, tex_size=64, max_tris=2048)
+    ref = refrast.render(s)
+    r = render.SceneRenderer(ctx, s)
+    ctx.set_mode(batched=True, tile=0, graphs=True)
+    shapes = []
+    for i in range(12):
+        r.render()
+        _compare(s, {"color": r.read_color(), "depth": r.read_depth()}, ref)
+        st = ctx.stats(0)
+        shapes.append((st["tile_size"], st["chunk"]))
+    assert len(set(shapes[:10])) == 5
+    assert shapes[10] == shapes[11] and shapes[10] in set(shapes[:10])
+    ctx.set_mode(batched=True, tile=8)

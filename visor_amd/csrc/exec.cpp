@@ -34,10 +34,11 @@ static uint32_t TileShiftFromEnv()
   if(t)
   {
     int v = atoi(t);
+    if(v == 8) return 3;
     if(v == 16) return 4;
     if(v == 32) return 5;
   }
-  return 3;
+  return 0;
 }
 
 Queue::Queue()
@@ -45,13 +46,25 @@ Queue::Queue()
   const char *m = getenv("VISOR_EXEC_MODE");
   if(m && !strcmp(m, "plain"))
     mode = ExecMode::Plain;
-  tiles.tile_shift = TileShiftFromEnv();
+  // VISOR_TILE=8/16/32 fixes the tile size; otherwise the autotuner chooses (VISOR_AUTOTUNE=0:
+  // fixed 8x8 tiles)
+  if(uint32_t sh = TileShiftFromEnv())
+  {
+    tiles.tile_shift = sh;
+    autotune = false;
+  }
+  if(const char *at = getenv("VISOR_AUTOTUNE"))
+    if(!strcmp(at, "0"))
+      autotune = false;
   // VISOR_CHUNK / VISOR_SMALL: raster module parameters (see TileConfig)
   if(const char *c = getenv("VISOR_CHUNK"))
   {
     int v = atoi(c);
     if(v >= 16 && v <= 64)
+    {
       tiles.chunk = (uint32_t)v;
+      autotune = false;
+    }
   }
   if(const char *c = getenv("VISOR_SMALL"))
   {
@@ -61,6 +74,8 @@ Queue::Queue()
   }
   const char *a = getenv("VISOR_SUBMIT");
   async = a && !strcmp(a, "async");
+  const char *g = getenv("VISOR_GRAPHS");
+  graphs = !(g && !strcmp(g, "0"));
 }
 
 Queue::~Queue()
@@ -84,6 +99,10 @@ Queue::~Queue()
   for(auto &e : ev)
     if(e)
       hipEventDestroy(e);
+  for(auto &g : graph_cache)
+    for(hipGraphExec_t x : g.exec)
+      if(x)
+        hipGraphExecDestroy(x);
   if(stream)
     hipStreamDestroy(stream);
 }
@@ -214,6 +233,7 @@ bool Queue::Decode(const VkCommandBuffer_T *cb, Submission &sub, std::string &er
   VkBuffer_T *vbs[4] = {};
   VkDeviceSize vb_off[4] = {};
   int pass = -1;
+  std::set<const VkDeviceMemory_T *> touched;  // memories written earlier in this command buffer
   auto pull = [&](size_t n) {
     const uint8_t *p = cur;
     cur += n;
@@ -273,12 +293,23 @@ bool Queue::Decode(const VkCommandBuffer_T *cb, Submission &sub, std::string &er
             op.frame.clear_dep = d->clearval[ci++];
           }
         }
-        AddMem(sub.uploads, op.frame.color->mem);
+        // A target whose memory the pass overwrites completely (cleared, image covering the
+        // whole allocation) and that nothing earlier in the submission touched needs no upload.
+        auto overwritten = [&](const VkImage_T *img, bool cleared) {
+          return cleared && img->offset == 0 && ImageBytes(img) >= img->mem->size &&
+                 std::find(sub.uploads.begin(), sub.uploads.end(), img->mem) == sub.uploads.end() &&
+                 !touched.count(img->mem);
+        };
+        if(!overwritten(op.frame.color, op.frame.clear_color))
+          AddMem(sub.uploads, op.frame.color->mem);
         AddMem(sub.downloads, op.frame.color->mem);
+        touched.insert(op.frame.color->mem);
         if(op.frame.depth)
         {
-          AddMem(sub.uploads, op.frame.depth->mem);
+          if(!overwritten(op.frame.depth, op.frame.clear_depth))
+            AddMem(sub.uploads, op.frame.depth->mem);
           AddMem(sub.downloads, op.frame.depth->mem);
+          touched.insert(op.frame.depth->mem);
         }
         sub.ops.push_back(std::move(op));
         pass = (int)sub.ops.size() - 1;
@@ -553,9 +584,80 @@ static int BitsFor(uint32_t n)
   return b;
 }
 
+// ---------------------------------------------------------------------------------------------
+// Autotuner. Every launch shape (tile size and its workgroup of (T/8)^2 waves, LDS chunk) gives
+// the same bytes, so a repeated pass shape is simply run with the next untried candidate and
+// timed by the stage events; after two rounds over the candidates the fastest is kept.
+
+static const TileConfig kTuneCandidates[] = {
+    {3, 64, 16}, {3, 32, 16}, {4, 64, 16}, {4, 32, 16}, {5, 64, 16},
+};
+static const int kTuneRounds = 2;
+
+uint64_t Queue::PassSignature(const std::vector<Op *> &passes) const
+{
+  uint64_t h = 1469598103934665603ull;
+  auto mix = [&](uint64_t v) {
+    h ^= v;
+    h *= 1099511628211ull;
+  };
+  for(Op *p : passes)
+  {
+    mix(p->frame.color->extent.width);
+    mix(p->frame.color->extent.height);
+    mix(p->frame.color->bytesPerPixel);
+    mix(p->frame.depth ? 1 : 0);
+    mix(p->draws.size());
+    uint64_t tris = 0;
+    for(const DrawDesc &dd : p->draws)
+      tris += dd.d.ntris;
+    mix(tris);
+  }
+  return h;
+}
+
+TileConfig Queue::ChooseConfig(uint64_t sig, bool *tuning)
+{
+  *tuning = false;
+  if(!autotune || mode != ExecMode::Batched)
+    return tiles;
+  TuneState &t = tune[sig];
+  if(t.done)
+    return t.best;
+  const size_t nc = sizeof(kTuneCandidates) / sizeof(kTuneCandidates[0]);
+  *tuning = true;
+  return kTuneCandidates[t.runs % nc];
+}
+
+void Queue::RecordTuning(uint64_t sig, const TileConfig &cfg, double ms)
+{
+  TuneState &t = tune[sig];
+  const size_t nc = sizeof(kTuneCandidates) / sizeof(kTuneCandidates[0]);
+  if(t.done)
+    return;
+  if(t.ms.size() < nc)
+    t.ms.assign(nc, 1e30);
+  size_t i = t.runs % nc;
+  t.ms[i] = std::min(t.ms[i], ms);
+  t.runs++;
+  if(t.runs >= nc * kTuneRounds)
+  {
+    size_t b = 0;
+    for(size_t k = 1; k < nc; k++)
+      if(t.ms[k] < t.ms[b])
+        b = k;
+    t.best = kTuneCandidates[b];
+    t.done = true;
+    (void)cfg;
+  }
+}
+
 VkResult Queue::RunPass(const std::vector<Op *> &passes, bool fused_clear, ExecStats &st)
 {
-  const uint32_t ts = tiles.tile_shift, T = 1u << ts;
+  bool tuning = false;
+  const uint64_t sig = PassSignature(passes);
+  const TileConfig cfg = ChooseConfig(sig, &tuning);
+  const uint32_t ts = cfg.tile_shift, T = 1u << ts;
   std::vector<VaFrame> frames;
   std::vector<VaDraw> draws;
   std::vector<std::shared_ptr<ShaderPair>> pairs;
@@ -615,7 +717,7 @@ VkResult Queue::RunPass(const std::vector<Op *> &passes, bool fused_clear, ExecS
   if(!pairs.empty())
   {
     std::string err;
-    mod = Device::Get().GetModule(pairs, err, tiles);
+    mod = Device::Get().GetModule(pairs, err, cfg);
     if(!mod)
     {
       LogError("pipeline module: %s", err.c_str());
@@ -651,11 +753,27 @@ VkResult Queue::RunPass(const std::vector<Op *> &passes, bool fused_clear, ExecS
     if(!ev[i])
       HIPCHK(hipEventCreate(&ev[i]));
 
-  HIPCHK(hipMemcpyAsync(d_frames, frames.data(), sizeof(VaFrame) * nframes, hipMemcpyHostToDevice, stream));
-  if(ndraws)
+  // the per-pass tables go up only when they changed (a re-submitted command buffer reuses them)
+  if(d_frames != up_frames_ptr || frames.size() != up_frames.size() ||
+     memcmp(frames.data(), up_frames.data(), sizeof(VaFrame) * nframes) != 0)
+  {
+    HIPCHK(hipMemcpyAsync(d_frames, frames.data(), sizeof(VaFrame) * nframes, hipMemcpyHostToDevice, stream));
+    up_frames = frames;
+    up_frames_ptr = d_frames;
+  }
+  if(ndraws && (d_draws != up_draws_ptr || draws.size() != up_draws.size() ||
+                memcmp(draws.data(), up_draws.data(), sizeof(VaDraw) * ndraws) != 0))
+  {
     HIPCHK(hipMemcpyAsync(d_draws, draws.data(), sizeof(VaDraw) * ndraws, hipMemcpyHostToDevice, stream));
-  if(nblocks)
+    up_draws = draws;
+    up_draws_ptr = d_draws;
+  }
+  if(nblocks && (d_blocks != up_blocks_ptr || block_draw != up_blocks))
+  {
     HIPCHK(hipMemcpyAsync(d_blocks, block_draw.data(), 4 * nblocks, hipMemcpyHostToDevice, stream));
+    up_blocks = block_draw;
+    up_blocks_ptr = d_blocks;
+  }
 
   // The pair buffers are sized by a capacity that only grows: the device work never waits for
   // the host. If a frame needs more pairs than the capacity, the raster launch writes nothing
@@ -663,7 +781,8 @@ VkResult Queue::RunPass(const std::vector<Op *> &passes, bool fused_clear, ExecS
   if(pair_cap == 0)
     pair_cap = std::max<uint32_t>(1u << 20, 2 * tri_total);
   uint32_t total_pairs = 0;
-  for(int attempt = 0;; attempt++)
+  int overflow_retries = 0;
+  for(;;)
   {
     if(!Grow(&d_keys, c_keys, 4ull * pair_cap) || !Grow(&d_vals, c_vals, 4ull * pair_cap) ||
        !Grow(&d_keys2, c_keys2, 4ull * pair_cap) || !Grow(&d_vals2, c_vals2, 4ull * pair_cap) ||
@@ -676,57 +795,151 @@ VkResult Queue::RunPass(const std::vector<Op *> &passes, bool fused_clear, ExecS
     VaCounters *ctr = (VaCounters *)d_ctr;
     uint64_t *part = (uint64_t *)d_part;
     uint32_t cap = pair_cap;
-    HIPCHK(va_fill_u32_async(d_ctr, sizeof(VaCounters) / 4, 0u, stream));
-    HIPCHK(hipEventRecord(ev[0], stream));
-    if(tri_total)
-    {
-      void *args[] = {&dd, &bd, &df, (void *)&ts, &rec, &cnt, &part};
-      HIPCHK(hipModuleLaunchKernel(mod->setup, nblocks, 1, 1, 256, 1, 1, 0, stream, args, nullptr));
-      HIPCHK(va_reduce_cols_async(part, nblocks, 4, (uint64_t *)&ctr->tris_in, stream));
-      HIPCHK(va_fill_u32_async(cnt + tri_total, 1, 0u, stream));
-      HIPCHK(va_scan_u32((uint32_t *)d_temp, cnt, offs, tri_total + 1, stream));
-    }
-    else
-      HIPCHK(va_fill_u32_async(offs, 1, 0u, stream));
-    HIPCHK(hipEventRecord(ev[1], stream));
-    HIPCHK(va_fill_u32_async(d_start, tile_total, 0u, stream));
-    HIPCHK(va_fill_u32_async(d_end, tile_total, 0u, stream));
-    uint32_t *bins = (uint32_t *)d_vals, *sorted_keys = (uint32_t *)d_keys;
-    if(tri_total)
-    {
-      uint32_t *keys = (uint32_t *)d_keys, *vals = (uint32_t *)d_vals;
-      void *eargs[] = {&rec, &offs, &tri_total, &cap, &dd, &df, (void *)&ts, &keys, &vals};
-      HIPCHK(hipModuleLaunchKernel(mod->emit, (tri_total + 255) / 256, 1, 1, 256, 1, 1, 0, stream, eargs, nullptr));
-      int in_alt = 0;
-      HIPCHK(va_sort_pairs((uint32_t *)d_stemp, keys, vals, (uint32_t *)d_keys2, (uint32_t *)d_vals2, total_dev, cap,
-                           BitsFor(tile_total), &in_alt, stream));
-      if(in_alt)
+    // The device steps of the pass in three stages (kernels only, so each stage can be captured
+    // into a graph): 0 setup + scan, 1 binning, 2 raster. The timing events go between stages,
+    // outside any graph.
+    const int sort_passes = (BitsFor(tile_total) + 7) / 8;
+    const bool in_alt = tri_total && (sort_passes & 1);
+    uint32_t *bins = in_alt ? (uint32_t *)d_vals2 : (uint32_t *)d_vals;
+    uint32_t *sorted_keys = in_alt ? (uint32_t *)d_keys2 : (uint32_t *)d_keys;
+    auto enqueue = [&](int stage) -> hipError_t {
+      hipError_t e;
+#define VA_TRY(x)               \
+  do                            \
+  {                             \
+    if((e = (x)) != hipSuccess) \
+      return e;                 \
+  } while(0)
+      if(stage == 0)
       {
-        sorted_keys = (uint32_t *)d_keys2;
-        bins = (uint32_t *)d_vals2;
+        VA_TRY(va_fill_u32_async(d_ctr, sizeof(VaCounters) / 4, 0u, stream));
+        if(tri_total)
+        {
+          void *args[] = {&dd, &bd, &df, (void *)&ts, &rec, &cnt, &part};
+          VA_TRY(hipModuleLaunchKernel(mod->setup, nblocks, 1, 1, 256, 1, 1, 0, stream, args, nullptr));
+          VA_TRY(va_reduce_cols_async(part, nblocks, 4, (uint64_t *)&ctr->tris_in, stream));
+          VA_TRY(va_fill_u32_async(cnt + tri_total, 1, 0u, stream));
+          VA_TRY(va_scan_u32((uint32_t *)d_temp, cnt, offs, tri_total + 1, stream));
+        }
+        else
+          VA_TRY(va_fill_u32_async(offs, 1, 0u, stream));
       }
-      uint32_t *st_ = (uint32_t *)d_start, *en = (uint32_t *)d_end;
-      void *rargs[] = {&sorted_keys, &total_dev, &cap, &st_, &en};
-      HIPCHK(hipModuleLaunchKernel(mod->ranges, (cap + 255) / 256, 1, 1, 256, 1, 1, 0, stream, rargs, nullptr));
-    }
-    HIPCHK(hipEventRecord(ev[2], stream));
+      else if(stage == 1)
+      {
+        VA_TRY(va_fill_u32_async(d_start, tile_total, 0u, stream));
+        VA_TRY(va_fill_u32_async(d_end, tile_total, 0u, stream));
+        if(tri_total)
+        {
+          uint32_t *keys = (uint32_t *)d_keys, *vals = (uint32_t *)d_vals;
+          void *eargs[] = {&rec, &offs, &tri_total, &cap, &dd, &df, (void *)&ts, &keys, &vals};
+          VA_TRY(hipModuleLaunchKernel(mod->emit, (tri_total + 255) / 256, 1, 1, 256, 1, 1, 0, stream, eargs, nullptr));
+          int alt = 0;
+          VA_TRY(va_sort_pairs((uint32_t *)d_stemp, keys, vals, (uint32_t *)d_keys2, (uint32_t *)d_vals2, total_dev, cap,
+                               BitsFor(tile_total), &alt, stream));
+          if((alt != 0) != in_alt)
+            return hipErrorUnknown;
+          uint32_t *st_ = (uint32_t *)d_start, *en = (uint32_t *)d_end;
+          void *rargs[] = {&sorted_keys, &total_dev, &cap, &st_, &en};
+          VA_TRY(hipModuleLaunchKernel(mod->ranges, (cap + 255) / 256, 1, 1, 256, 1, 1, 0, stream, rargs, nullptr));
+        }
+      }
+      else
+      {
+        uint32_t *st_ = (uint32_t *)d_start, *en = (uint32_t *)d_end;
+        uint32_t use_fused = fused_clear ? 1u : 0u;
+        void *args[] = {&dd, &df, (void *)&nframes, &rec, &bins, &st_, &en, &tile_total, &use_fused, &part, &total_dev, &cap};
+        hipFunction_t k = mod->raster[ts - 3];
+        uint32_t threads = (T / 8) * (T / 8) * 64;
+        VA_TRY(hipModuleLaunchKernel(k, tile_total, 1, 1, threads, 1, 1, 0, stream, args, nullptr));
+        VA_TRY(va_reduce_cols_async(part, tile_total, 4, (uint64_t *)&ctr->pixels_written, stream));
+      }
+#undef VA_TRY
+      return hipSuccess;
+    };
+
+    const GraphSet *gs = nullptr;
+    if(graphs && mode == ExecMode::Batched)
     {
-      uint32_t *st_ = (uint32_t *)d_start, *en = (uint32_t *)d_end;
-      uint32_t use_fused = fused_clear ? 1u : 0u;
-      void *args[] = {&dd, &df, (void *)&nframes, &rec, &bins, &st_, &en, &tile_total, &use_fused, &part, &total_dev, &cap};
-      hipFunction_t k = mod->raster[ts - 3];
-      uint32_t threads = (T / 8) * (T / 8) * 64;
-      HIPCHK(hipModuleLaunchKernel(k, tile_total, 1, 1, threads, 1, 1, 0, stream, args, nullptr));
-      HIPCHK(hipEventRecord(ev[3], stream));
-      HIPCHK(va_reduce_cols_async(part, tile_total, 4, (uint64_t *)&ctr->pixels_written, stream));
+      // the graphs' kernel arguments are fixed by these values; anything else is table content
+      GraphKey key;
+      memset(&key, 0, sizeof(key));
+      key.mod = mod.get();
+      key.dims[0] = ts;
+      key.dims[1] = tri_total;
+      key.dims[2] = nblocks;
+      key.dims[3] = tile_total;
+      key.dims[4] = nframes;
+      key.dims[5] = cap;
+      key.dims[6] = fused_clear ? 1u : 0u;
+      void *bufs[] = {d_frames, d_draws, d_rec, d_count, d_offs, d_keys, d_vals, d_keys2, d_vals2,
+                      d_stemp, d_temp, d_start, d_end, d_ctr, d_blocks, d_part};
+      memcpy(key.bufs, bufs, sizeof(bufs));
+      for(size_t i = 0; i < graph_cache.size(); i++)
+        if(memcmp(&graph_cache[i].key, &key, sizeof(key)) == 0)
+        {
+          std::rotate(graph_cache.begin(), graph_cache.begin() + i, graph_cache.begin() + i + 1);
+          gs = &graph_cache[0];
+          break;
+        }
+      if(!gs)
+      {
+        GraphSet set;
+        set.key = key;
+        hipError_t err = hipSuccess;
+        for(int stage = 0; stage < 3 && err == hipSuccess; stage++)
+        {
+          hipError_t e = hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal);
+          hipError_t er = e == hipSuccess ? enqueue(stage) : e;
+          hipGraph_t g = nullptr;
+          hipError_t ec = e == hipSuccess ? hipStreamEndCapture(stream, &g) : e;
+          err = er != hipSuccess ? er : ec;
+          if(err == hipSuccess && g)
+            err = hipGraphInstantiate(&set.exec[stage], g, nullptr, nullptr, 0);
+          else if(err == hipSuccess)
+            err = hipErrorUnknown;
+          if(g)
+            hipGraphDestroy(g);
+        }
+        if(err != hipSuccess)
+        {
+          for(hipGraphExec_t x : set.exec)
+            if(x)
+              hipGraphExecDestroy(x);
+          (void)hipGetLastError();
+          LogError("graph capture failed (%s); running passes kernel by kernel", hipGetErrorString(err));
+          graphs = false;
+        }
+        else
+        {
+          graph_cache.insert(graph_cache.begin(), set);
+          if(graph_cache.size() > 8)
+          {
+            for(hipGraphExec_t x : graph_cache.back().exec)
+              hipGraphExecDestroy(x);
+            graph_cache.pop_back();
+          }
+          gs = &graph_cache[0];
+        }
+      }
     }
+    for(int stage = 0; stage < 3; stage++)
+    {
+      HIPCHK(hipEventRecord(ev[stage], stream));
+      if(gs)
+        HIPCHK(hipGraphLaunch(gs->exec[stage], stream));
+      else
+        HIPCHK(enqueue(stage));
+    }
+    HIPCHK(hipEventRecord(ev[3], stream));
+    if(gs)
+      st.graph_launches++;
     HIPCHK(hipMemcpyAsync(h_pinned + 64, d_ctr, sizeof(VaCounters), hipMemcpyDeviceToHost, stream));
     HIPCHK(hipMemcpyAsync(h_pinned, total_dev, 4, hipMemcpyDeviceToHost, stream));
     HIPCHK(hipStreamSynchronize(stream));
     total_pairs = h_pinned[0];
     if(total_pairs <= pair_cap)
       break;
-    if(attempt > 0)
+    if(overflow_retries++ > 0)
     {
       LogError("bin capacity did not converge (%u pairs)", total_pairs);
       return VK_ERROR_OUT_OF_DEVICE_MEMORY;
@@ -736,16 +949,27 @@ VkResult Queue::RunPass(const std::vector<Op *> &passes, bool fused_clear, ExecS
   VaCounters c;
   memcpy(&c, h_pinned + 64, sizeof(c));
   c.draws = ndraws;
-  float ms = 0.0f;
-  if(tri_total)
+  st.last_cfg = cfg;
   {
-    hipEventElapsedTime(&ms, ev[0], ev[1]);
-    st.setup_ms += ms;
-    hipEventElapsedTime(&ms, ev[1], ev[2]);
-    st.bin_ms += ms;
+    float t01 = 0.0f, t12 = 0.0f, t23 = 0.0f;
+    hipError_t e1 = hipEventElapsedTime(&t01, ev[0], ev[1]);
+    hipError_t e2 = hipEventElapsedTime(&t12, ev[1], ev[2]);
+    hipError_t e3 = hipEventElapsedTime(&t23, ev[2], ev[3]);
+    if(e1 != hipSuccess || e2 != hipSuccess || e3 != hipSuccess)
+    {
+      (void)hipGetLastError();
+      LogError("stage timing unavailable: %s", hipGetErrorString(e1 != hipSuccess ? e1 : (e2 != hipSuccess ? e2 : e3)));
+    }
+    else
+    {
+      st.setup_ms += t01;
+      st.bin_ms += t12;
+      st.raster_ms += t23;
+      if(tuning)
+        RecordTuning(sig, cfg, (double)t01 + t12 + t23);
+    }
   }
-  hipEventElapsedTime(&ms, ev[2], ev[3]);
-  st.raster_ms += ms;
+
   st.raster_launches++;
   st.total_tris += tri_total;
   st.total_pairs += total_pairs;
@@ -815,7 +1039,11 @@ VkResult Queue::Execute(std::vector<Submission> &subs)
     if(e != hipSuccess && res == VK_SUCCESS)
       res = HipToVk(e);
     float ms = 0.0f;
-    hipEventElapsedTime(&ms, e0, e1);
+    if(hipEventElapsedTime(&ms, e0, e1) != hipSuccess)
+    {
+      ms = 0.0f;
+      (void)hipGetLastError();
+    }
     st.gpu_ms_total = ms;
     hipEventDestroy(e0);
     hipEventDestroy(e1);
@@ -828,6 +1056,7 @@ VkResult Queue::Execute(std::vector<Submission> &subs)
     accum_stats.total_tris += st.total_tris;
     accum_stats.total_pairs += st.total_pairs;
     accum_stats.total_tiles += st.total_tiles;
+    accum_stats.graph_launches += st.graph_launches;
     accum_stats.ctr.pixels_written += st.ctr.pixels_written;
     accum_stats.ctr.depth_passed += st.ctr.depth_passed;
     accum_stats.ctr.tris_in += st.ctr.tris_in;

@@ -1269,14 +1269,28 @@ int vh_set_exec_mode(VkQueue queue, int mode)
   return 0;
 }
 
+int vh_set_graph_mode(VkQueue queue, int on)
+{
+  if(!queue || (on != 0 && on != 1))
+    return -1;
+  queue->q->graphs = on != 0;
+  return 0;
+}
+
 int vh_set_tile_size(VkQueue queue, int tile)
 {
   if(!queue)
     return -1;
-  if(tile == 8) queue->q->tiles.tile_shift = 3;
-  else if(tile == 16) queue->q->tiles.tile_shift = 4;
-  else if(tile == 32) queue->q->tiles.tile_shift = 5;
-  else return -1;
+  if(tile == 0)
+  {
+    queue->q->autotune = true;
+    return 0;
+  }
+  uint32_t shift = tile == 8 ? 3u : (tile == 16 ? 4u : (tile == 32 ? 5u : 0u));
+  if(!shift)
+    return -1;
+  queue->q->tiles.tile_shift = shift;
+  queue->q->autotune = false;  // an explicit launch shape replaces the autotuner's choice
   return 0;
 }
 
@@ -1311,6 +1325,9 @@ int vh_get_stats(VkQueue queue, int which, VhStats *out)
   out->depth_passed = s.ctr.depth_passed;
   out->shaded_blend = s.ctr.shaded_blend;
   out->shaded_dwrite = s.ctr.shaded_dwrite;
+  out->graph_launches = s.graph_launches;
+  out->tile_size = 1ull << s.last_cfg.tile_shift;
+  out->chunk = s.last_cfg.chunk;
   return 0;
 }
 

@@ -314,23 +314,6 @@ struct Submission
   double t_submit = 0.0;
 };
 
-struct ExecStats
-{
-  double gpu_ms_total = 0.0;    // hipEvent time around the whole submission's device work
-  double raster_ms = 0.0;       // hipEvent time around the raster kernel(s)
-  double setup_ms = 0.0;
-  double bin_ms = 0.0;
-  uint64_t raster_launches = 0;
-  uint64_t total_tris = 0, total_pairs = 0, total_tiles = 0;
-  VaCounters ctr{};
-};
-
-enum class ExecMode
-{
-  Plain = 0,    // per draw: setup, bin, raster, host sync (the reference's blocking DrawTriangles)
-  Batched = 1,  // every draw of a render pass in one setup/bin/raster sequence, fused clears
-};
-
 // Launch shape of the raster stage. tile_shift selects the kernel at launch; chunk (bin
 // entries staged in LDS per round, 16..64) and small (largest bbox-in-block overlap tested
 // triangle-parallel) are compile-time parameters of the pipeline module.
@@ -344,6 +327,26 @@ struct TileConfig
     return tile_shift == o.tile_shift && chunk == o.chunk && small == o.small;
   }
 };
+
+struct ExecStats
+{
+  double gpu_ms_total = 0.0;    // hipEvent time around the whole submission's device work
+  double raster_ms = 0.0;       // hipEvent time around the raster kernel(s)
+  double setup_ms = 0.0;
+  double bin_ms = 0.0;
+  uint64_t raster_launches = 0;
+  uint64_t total_tris = 0, total_pairs = 0, total_tiles = 0;
+  uint64_t graph_launches = 0;  // passes replayed from a captured graph
+  TileConfig last_cfg;          // launch shape of the last pass
+  VaCounters ctr{};
+};
+
+enum class ExecMode
+{
+  Plain = 0,    // per draw: setup, bin, raster, host sync (the reference's blocking DrawTriangles)
+  Batched = 1,  // every draw of a render pass in one setup/bin/raster sequence, fused clears
+};
+
 
 class Device
 {
@@ -375,16 +378,31 @@ public:
   ~Queue();
   VkResult Submit(uint32_t count, const VkSubmitInfo *infos, VkFence fence);
   VkResult WaitIdle();
+  // several submissions' render passes in one batched launch sequence (serving)
+  VkResult ExecuteBatch(std::vector<Submission *> &subs, ExecStats &st);
   ExecStats last_stats;
   ExecStats accum_stats;
   ExecMode mode = ExecMode::Batched;
   TileConfig tiles;
   bool async = false;
+  bool graphs = true;  // batched passes run as a captured, replayed HIP graph
+  bool autotune = true;  // batched passes pick their launch shape per pass signature
   bool Decode(const VkCommandBuffer_T *cb, Submission &sub, std::string &err);
 
 private:
   VkResult Execute(std::vector<Submission> &subs);
   VkResult RunPass(const std::vector<Op *> &passes, bool fused_clear, ExecStats &st);
+  uint64_t PassSignature(const std::vector<Op *> &passes) const;
+  TileConfig ChooseConfig(uint64_t sig, bool *tuning);
+  void RecordTuning(uint64_t sig, const TileConfig &cfg, double ms);
+  struct TuneState
+  {
+    std::vector<double> ms;  // best time per candidate
+    size_t runs = 0;
+    bool done = false;
+    TileConfig best;
+  };
+  std::map<uint64_t, TuneState> tune;
   VkResult RunCopy(const CopyDesc &c);
   VkResult ClearFrame(const FrameDesc &f);
   bool Grow(void **p, size_t &cap, size_t need);
@@ -404,6 +422,24 @@ private:
   void *d_stemp = nullptr;  // radix sort histograms
   size_t c_stemp = 0;
   uint32_t pair_cap = 0;    // (tile, primitive) pair capacity of the bin buffers
+  // last uploaded per-pass tables (skipped when a submission repeats them)
+  std::vector<VaFrame> up_frames;
+  std::vector<VaDraw> up_draws;
+  std::vector<uint32_t> up_blocks;
+  void *up_frames_ptr = nullptr, *up_draws_ptr = nullptr, *up_blocks_ptr = nullptr;
+  // the captured device sequence of the last batched pass, replayed while its key matches
+  struct GraphKey
+  {
+    const void *mod;
+    uint32_t dims[8];
+    void *bufs[16];
+  };
+  struct GraphSet
+  {
+    GraphKey key;
+    hipGraphExec_t exec[3] = {nullptr, nullptr, nullptr};  // setup + scan, binning, raster
+  };
+  std::vector<GraphSet> graph_cache;  // most recent first, at most 8
   size_t c_frames = 0, c_draws = 0, c_rec = 0, c_count = 0, c_offs = 0, c_keys = 0, c_vals = 0, c_keys2 = 0,
          c_vals2 = 0, c_start = 0, c_end = 0, c_ctr = 0, c_temp = 0;
   uint32_t *h_pinned = nullptr;  // small pinned staging for readbacks

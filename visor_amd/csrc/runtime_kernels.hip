@@ -142,8 +142,11 @@ extern "C" hipError_t va_scan_u32(uint32_t *temp, const uint32_t *in, uint32_t *
   if(n == 0)
     return hipSuccess;
   const u32 nb = scan_blocks(n);
+  (void)hipGetLastError();  // report this launch's error only
   hipLaunchKernelGGL(va_scan_reduce_kernel, dim3(nb), dim3(RK_SCAN_T), 0, stream, in, n, temp);
+  (void)hipGetLastError();  // report this launch's error only
   hipLaunchKernelGGL(va_scan_top_kernel, dim3(1), dim3(1024), 0, stream, temp, nb);
+  (void)hipGetLastError();  // report this launch's error only
   hipLaunchKernelGGL(va_scan_down_kernel, dim3(nb), dim3(RK_SCAN_T), 0, stream, in, n, (const u32 *)temp, out);
   return hipGetLastError();
 }
@@ -269,11 +272,13 @@ extern "C" hipError_t va_sort_pairs(uint32_t *temp, uint32_t *keys, uint32_t *va
   u32 *ki = keys, *vi = vals, *ko = keys_alt, *vo = vals_alt;
   for(int shift = 0; shift < bits; shift += 8)
   {
+    (void)hipGetLastError();  // report this launch's error only
     hipLaunchKernelGGL(va_sort_hist_kernel, dim3(nbs), dim3(RK_SORT_T), 0, stream, (const u32 *)ki, n_dev, cap,
                        (u32)shift, hist, nbs);
     hipError_t e = va_scan_u32(scan_tmp, hist, hist, nh, stream);
     if(e != hipSuccess)
       return e;
+    (void)hipGetLastError();  // report this launch's error only
     hipLaunchKernelGGL(va_sort_scatter_kernel, dim3(nbs), dim3(RK_SORT_T), 0, stream, (const u32 *)ki,
                        (const u32 *)vi, ko, vo, n_dev, cap, (u32)shift, (const u32 *)hist, nbs);
     u32 *t = ki; ki = ko; ko = t;
@@ -310,6 +315,7 @@ extern "C" hipError_t va_fill_u32_async(void *p, uint64_t count, uint32_t value,
 {
   if(count == 0)
     return hipSuccess;
+  (void)hipGetLastError();  // report this launch's error only
   hipLaunchKernelGGL(va_fill32_kernel, dim3(fill_blocks(count)), dim3(256), 0, stream, (uint32_t *)p, count, value);
   return hipGetLastError();
 }
@@ -318,6 +324,7 @@ extern "C" hipError_t va_fill_u64_async(void *p, uint64_t count, uint32_t lo, ui
 {
   if(count == 0)
     return hipSuccess;
+  (void)hipGetLastError();  // report this launch's error only
   hipLaunchKernelGGL(va_fill64_kernel, dim3(fill_blocks(count)), dim3(256), 0, stream, (uint2 *)p, count,
                      make_uint2(lo, hi));
   return hipGetLastError();
@@ -361,6 +368,7 @@ extern "C" hipError_t va_reduce_cols_async(const uint64_t *part, uint32_t rows, 
   uint32_t blocks = (rows + 1023) / 1024;
   if(blocks < 1) blocks = 1;
   if(blocks > 64) blocks = 64;
+  (void)hipGetLastError();  // report this launch's error only
   hipLaunchKernelGGL(va_reduce4_kernel, dim3(blocks), dim3(256), 0, stream, part, rows, out);
   return hipGetLastError();
 }

@@ -152,8 +152,10 @@ class Context:
         self.icd.check(self.icd.vkCreateShaderModule(self.device, C.byref(sci), None, C.byref(m)), "vkCreateShaderModule")
         return m
 
-    def set_mode(self, batched=True, tile=None, async_submit=None):
+    def set_mode(self, batched=True, tile=None, async_submit=None, graphs=None):
         self.icd.vh_set_exec_mode(self.queue, 1 if batched else 0)
+        if graphs is not None:
+            self.icd.vh_set_graph_mode(self.queue, 1 if graphs else 0)
         if tile is not None:
             self.icd.vh_set_tile_size(self.queue, tile)
         if async_submit is not None:

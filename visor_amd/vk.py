@@ -228,7 +228,18 @@ class VhStats(C.Structure):
     _fields_ = [("gpu_ms", C.c_double), ("raster_ms", C.c_double), ("setup_ms", C.c_double), ("bin_ms", C.c_double),
                 ("raster_launches", u64), ("triangles", u64), ("bin_pairs", u64), ("tiles", u64), ("draws", u64),
                 ("tris_in", u64), ("tris_out", u64), ("blocks", u64), ("pixels_tested", u64), ("pixels_written", u64),
-                ("depth_passed", u64), ("shaded_blend", u64), ("shaded_dwrite", u64)]
+                ("depth_passed", u64), ("shaded_blend", u64), ("shaded_dwrite", u64),
+                ("graph_launches", u64), ("tile_size", u64), ("chunk", u64)]
+
+
+class VhServerConfig(C.Structure):
+    _fields_ = [("hbm_budget_bytes", u64), ("slots", u32), ("max_batch", u32)]
+
+
+class VhServerStats(C.Structure):
+    _fields_ = [("submitted", u64), ("completed", u64), ("batches", u64), ("fragments", u64), ("shaded", u64),
+                ("max_inflight_bytes", u64), ("busy_ms", C.c_double), ("mean_batch", C.c_double),
+                ("p50_ms", C.c_double), ("p99_ms", C.c_double), ("mean_ms", C.c_double), ("max_ms", C.c_double)]
 
 
 P = C.POINTER
@@ -306,6 +317,13 @@ EXT_PROTOS = [
     ("vh_last_error", cp, []),
     ("vh_set_exec_mode", C.c_int, [Handle, C.c_int]),
     ("vh_set_tile_size", C.c_int, [Handle, C.c_int]),
+    ("vh_set_graph_mode", C.c_int, [Handle, C.c_int]),
+    ("vh_server_create", i32, [Handle, P(VhServerConfig), P(vp)]),
+    ("vh_server_destroy", None, [vp]),
+    ("vh_server_submit", i32, [vp, Handle, P(u64)]),
+    ("vh_server_wait", i32, [vp, u64, u64, P(C.c_double)]),
+    ("vh_server_stats", C.c_int, [vp, P(VhServerStats)]),
+    ("vh_server_reset_stats", C.c_int, [vp]),
     ("vh_set_async_submit", C.c_int, [Handle, C.c_int]),
     ("vh_get_stats", C.c_int, [Handle, C.c_int, P(VhStats)]),
     ("vh_reset_stats", C.c_int, [Handle]),
