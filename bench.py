@@ -87,6 +87,7 @@ def main():
                     "its 5 launch shapes twice")
     ap.add_argument("--plain-steps", type=int, default=3)
     ap.add_argument("--tile", type=int, default=0, help="8, 16 or 32 fixes the tile size; 0 = autotuned")
+    ap.add_argument("--chunk", type=int, default=0, help="16..64 fixes the LDS chunk (with --tile); 0 = default")
     ap.add_argument("--cpu-draws", type=int, default=128, help="draws in the CPU-baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
@@ -115,6 +116,8 @@ def main():
               file=sys.stderr)
         sys.exit(2)
 
+    if args.chunk:
+        os.environ["VISOR_CHUNK"] = str(args.chunk)  # read when the queue is created
     from visor_amd import render, scenes
 
     try:
