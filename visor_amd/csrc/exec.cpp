@@ -90,7 +90,7 @@ Queue::~Queue()
     worker.join();
   }
   void *bufs[] = {d_frames, d_draws, d_rec, d_count, d_offs, d_keys, d_vals, d_keys2, d_vals2, d_start, d_end, d_ctr, d_temp,
-                  d_blocks, d_part, d_stemp};
+                  d_blocks, d_part, d_stemp, d_einfo};
   for(void *b : bufs)
     if(b)
       hipFree(b);
@@ -741,8 +741,9 @@ VkResult Queue::RunPass(const std::vector<Op *> &passes, bool fused_clear, ExecS
   const uint32_t nblocks = (uint32_t)block_draw.size();
   if(!Grow(&d_frames, c_frames, sizeof(VaFrame) * nframes) || !Grow(&d_draws, c_draws, sizeof(VaDraw) * std::max(ndraws, 1u)) ||
      !Grow(&d_rec, c_rec, (size_t)stride * 4 * std::max(tri_total, 1u)) || !Grow(&d_count, c_count, 4ull * (tri_total + 1)) ||
-     !Grow(&d_offs, c_offs, 4ull * (tri_total + 1)) || !Grow(&d_start, c_start, 4ull * tile_total) ||
-     !Grow(&d_end, c_end, 4ull * tile_total) || !Grow(&d_ctr, c_ctr, sizeof(VaCounters)) ||
+     !Grow(&d_offs, c_offs, 4ull * (tri_total + 1)) || !Grow(&d_start, c_start, 4ull * (tile_total + 1)) ||
+     !Grow(&d_einfo, c_einfo, 8ull * std::max(tri_total, 1u)) ||
+     !Grow(&d_ctr, c_ctr, sizeof(VaCounters)) ||
      !Grow(&d_blocks, c_blocks, 4ull * std::max<size_t>(nblocks, 1)) ||
      !Grow(&d_part, c_part, 32ull * (std::max<size_t>(nblocks, tile_total) + 1)) ||
      !Grow(&d_temp, c_temp, 4 * va_scan_temp_words(tri_total + 1)))
@@ -812,40 +813,46 @@ VkResult Queue::RunPass(const std::vector<Op *> &passes, bool fused_clear, ExecS
   } while(0)
       if(stage == 0)
       {
-        VA_TRY(va_fill_u32_async(d_ctr, sizeof(VaCounters) / 4, 0u, stream));
         if(tri_total)
         {
-          void *args[] = {&dd, &bd, &df, (void *)&ts, &rec, &cnt, &part};
+          // the setup kernel zeroes the counters and the scan's last count itself
+          uint2 *einfo = (uint2 *)d_einfo;
+          uint32_t *ctr_words = (uint32_t *)d_ctr;
+          void *args[] = {&dd, &bd, &df, (void *)&ts, &rec, &cnt, &part, &einfo, &tri_total, &ctr_words};
           VA_TRY(hipModuleLaunchKernel(mod->setup, nblocks, 1, 1, 256, 1, 1, 0, stream, args, nullptr));
           VA_TRY(va_reduce_cols_async(part, nblocks, 4, (uint64_t *)&ctr->tris_in, stream));
-          VA_TRY(va_fill_u32_async(cnt + tri_total, 1, 0u, stream));
           VA_TRY(va_scan_u32((uint32_t *)d_temp, cnt, offs, tri_total + 1, stream));
         }
         else
+        {
+          VA_TRY(va_fill_u32_async(d_ctr, sizeof(VaCounters) / 4, 0u, stream));
           VA_TRY(va_fill_u32_async(offs, 1, 0u, stream));
+        }
       }
       else if(stage == 1)
       {
-        VA_TRY(va_fill_u32_async(d_start, tile_total, 0u, stream));
-        VA_TRY(va_fill_u32_async(d_end, tile_total, 0u, stream));
         if(tri_total)
         {
           uint32_t *keys = (uint32_t *)d_keys, *vals = (uint32_t *)d_vals;
-          void *eargs[] = {&rec, &offs, &tri_total, &cap, &dd, &df, (void *)&ts, &keys, &vals};
+          uint2 *einfo = (uint2 *)d_einfo;
+          void *eargs[] = {&einfo, &offs, &tri_total, &cap, &keys, &vals};
           VA_TRY(hipModuleLaunchKernel(mod->emit, (tri_total + 255) / 256, 1, 1, 256, 1, 1, 0, stream, eargs, nullptr));
           int alt = 0;
           VA_TRY(va_sort_pairs((uint32_t *)d_stemp, keys, vals, (uint32_t *)d_keys2, (uint32_t *)d_vals2, total_dev, cap,
                                BitsFor(tile_total), &alt, stream));
           if((alt != 0) != in_alt)
             return hipErrorUnknown;
-          uint32_t *st_ = (uint32_t *)d_start, *en = (uint32_t *)d_end;
-          void *rargs[] = {&sorted_keys, &total_dev, &cap, &st_, &en};
-          VA_TRY(hipModuleLaunchKernel(mod->ranges, (cap + 255) / 256, 1, 1, 256, 1, 1, 0, stream, rargs, nullptr));
+          uint32_t *st_ = (uint32_t *)d_start;
+          void *rargs[] = {&sorted_keys, &total_dev, &cap, &tile_total, &st_};
+          VA_TRY(hipModuleLaunchKernel(mod->ranges, (tile_total + 256) / 256, 1, 1, 256, 1, 1, 0, stream, rargs, nullptr));
         }
+        else
+          VA_TRY(va_fill_u32_async(d_start, tile_total + 1, 0u, stream));
       }
       else
       {
-        uint32_t *st_ = (uint32_t *)d_start, *en = (uint32_t *)d_end;
+        // tile t's bin is [start[t], start[t + 1])
+        uint32_t *st_ = (uint32_t *)d_start, *en = (uint32_t *)d_start + 1;
         uint32_t use_fused = fused_clear ? 1u : 0u;
         void *args[] = {&dd, &df, (void *)&nframes, &rec, &bins, &st_, &en, &tile_total, &use_fused, &part, &total_dev, &cap};
         hipFunction_t k = mod->raster[ts - 3];
@@ -872,7 +879,7 @@ VkResult Queue::RunPass(const std::vector<Op *> &passes, bool fused_clear, ExecS
       key.dims[5] = cap;
       key.dims[6] = fused_clear ? 1u : 0u;
       void *bufs[] = {d_frames, d_draws, d_rec, d_count, d_offs, d_keys, d_vals, d_keys2, d_vals2,
-                      d_stemp, d_temp, d_start, d_end, d_ctr, d_blocks, d_part};
+                      d_stemp, d_temp, d_start, d_einfo, d_ctr, d_blocks, d_part};
       memcpy(key.bufs, bufs, sizeof(bufs));
       for(size_t i = 0; i < graph_cache.size(); i++)
         if(memcmp(&graph_cache[i].key, &key, sizeof(key)) == 0)

@@ -421,6 +421,8 @@ private:
   size_t c_part = 0;
   void *d_stemp = nullptr;  // radix sort histograms
   size_t c_stemp = 0;
+  void *d_einfo = nullptr;  // per-triangle pair-emission descriptors
+  size_t c_einfo = 0;
   uint32_t pair_cap = 0;    // (tile, primitive) pair capacity of the bin buffers
   // last uploaded per-pass tables (skipped when a submission repeats them)
   std::vector<VaFrame> up_frames;

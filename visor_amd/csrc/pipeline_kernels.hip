@@ -88,10 +88,23 @@ VA_INLINE u32 va_index(const VaDraw &d, u32 i)
 
 // One workgroup covers 256 primitives of a single draw (block_draw maps workgroup -> draw), so
 // the draw's state (push constants, vertex layout, descriptors) is read with scalar loads.
+// Records are assembled in LDS (word w of the thread's record at s_rec[w * 257 + thread]: no
+// bank conflicts) and written out as the block's contiguous record range with coalesced stores.
+// emit_info[g] = (first tile key, tiles per bbox row | frame tiles per row << 16) for va_emit.
+// Workgroup 0 also zeroes the frame counters and the count-scan's last element.
 extern "C" __global__ __launch_bounds__(256) void va_setup(
     const VaDraw *__restrict__ draws, const u32 *__restrict__ block_draw, const VaFrame *__restrict__ frames,
-    u32 tile_shift, u32 *__restrict__ rec, u32 *__restrict__ tile_count, u64 *__restrict__ part)
+    u32 tile_shift, u32 *__restrict__ rec, u32 *__restrict__ tile_count, u64 *__restrict__ part,
+    uint2 *__restrict__ emit_info, u32 total_tris, u32 *__restrict__ counters)
 {
+  __shared__ u32 s_rec[VA_REC_STRIDE * 257];
+  if(blockIdx.x == 0u)
+  {
+    if(threadIdx.x < (u32)(sizeof(VaCounters) / 4))
+      counters[threadIdx.x] = 0u;
+    if(threadIdx.x == 0u)
+      tile_count[total_tris] = 0u;
+  }
   const u32 di = __builtin_amdgcn_readfirstlane(block_draw[blockIdx.x]);
   const VaDraw &d = draws[di];
   const u32 prim = (blockIdx.x - d.block_base) * 256u + threadIdx.x;
@@ -157,41 +170,55 @@ extern "C" __global__ __launch_bounds__(256) void va_setup(
         if(maxx > minx && maxy > miny)
         {
           n_tested = (u64)(maxx - minx) * (u64)(maxy - miny);
-          u32 *r = rec + (u64)g * VA_REC_STRIDE;
+          u32 *r = s_rec + threadIdx.x;  // word w at r[w * 257]
           const u32 bm = (u32)barymul;
-          r[VA_R_X0] = (u32)X[0];
-          r[VA_R_Y0] = (u32)Y[0];
+          r[(VA_R_X0) * 257] = (u32)X[0];
+          r[(VA_R_Y0) * 257] = (u32)Y[0];
           // barymul folded into the edge coefficients: exact in wrapping arithmetic
-          r[VA_R_ABX] = (u32)(X[1] - X[0]) * bm;
-          r[VA_R_ABY] = (u32)(Y[1] - Y[0]) * bm;
-          r[VA_R_ACX] = (u32)(X[2] - X[0]) * bm;
-          r[VA_R_ACY] = (u32)(Y[2] - Y[0]) * bm;
-          r[VA_R_AREA] = (u32)flipped;
-          r[VA_R_INVAREA] = va_f2u(1.0f / (float)flipped);
+          r[(VA_R_ABX) * 257] = (u32)(X[1] - X[0]) * bm;
+          r[(VA_R_ABY) * 257] = (u32)(Y[1] - Y[0]) * bm;
+          r[(VA_R_ACX) * 257] = (u32)(X[2] - X[0]) * bm;
+          r[(VA_R_ACY) * 257] = (u32)(Y[2] - Y[0]) * bm;
+          r[(VA_R_AREA) * 257] = (u32)flipped;
+          r[(VA_R_INVAREA) * 257] = va_f2u(1.0f / (float)flipped);
           const float iw0 = 1.0f / pos[0][3], iw1 = 1.0f / pos[1][3], iw2 = 1.0f / pos[2][3];
-          r[VA_R_Z0 + 0] = va_f2u(pos[0][2] * iw0);
-          r[VA_R_Z0 + 1] = va_f2u(pos[1][2] * iw1);
-          r[VA_R_Z0 + 2] = va_f2u(pos[2][2] * iw2);
-          r[VA_R_IW0 + 0] = va_f2u(iw0);
-          r[VA_R_IW0 + 1] = va_f2u(iw1);
-          r[VA_R_IW0 + 2] = va_f2u(iw2);
-          r[VA_R_BBMIN] = (u32)minx | ((u32)miny << 16);
-          r[VA_R_BBMAX] = (u32)maxx | ((u32)maxy << 16);
-          r[VA_R_DRAW] = di;
+          r[(VA_R_Z0 + 0) * 257] = va_f2u(pos[0][2] * iw0);
+          r[(VA_R_Z0 + 1) * 257] = va_f2u(pos[1][2] * iw1);
+          r[(VA_R_Z0 + 2) * 257] = va_f2u(pos[2][2] * iw2);
+          r[(VA_R_IW0 + 0) * 257] = va_f2u(iw0);
+          r[(VA_R_IW0 + 1) * 257] = va_f2u(iw1);
+          r[(VA_R_IW0 + 2) * 257] = va_f2u(iw2);
+          r[(VA_R_BBMIN) * 257] = (u32)minx | ((u32)miny << 16);
+          r[(VA_R_BBMAX) * 257] = (u32)maxx | ((u32)maxy << 16);
+          r[(VA_R_DRAW) * 257] = di;
 #pragma unroll
           for(int j = 0; j < VA_MAXNV; j++)
           {
-            r[VA_REC_HDR + 3 * j + 0] = va_f2u(vary[0][j]);
-            r[VA_REC_HDR + 3 * j + 1] = va_f2u(vary[1][j]);
-            r[VA_REC_HDR + 3 * j + 2] = va_f2u(vary[2][j]);
+            r[(VA_REC_HDR + 3 * j + 0) * 257] = va_f2u(vary[0][j]);
+            r[(VA_REC_HDR + 3 * j + 1) * 257] = va_f2u(vary[1][j]);
+            r[(VA_REC_HDR + 3 * j + 2) * 257] = va_f2u(vary[2][j]);
           }
           const u32 tx0 = (u32)minx >> tile_shift, tx1 = (u32)(maxx - 1) >> tile_shift;
           const u32 ty0 = (u32)miny >> tile_shift, ty1 = (u32)(maxy - 1) >> tile_shift;
           cnt = (tx1 - tx0 + 1u) * (ty1 - ty0 + 1u);
+          emit_info[g] = make_uint2(fr.tile_base + ty0 * fr.tiles_x + tx0, (tx1 - tx0 + 1u) | (fr.tiles_x << 16));
         }
       }
     }
     tile_count[g] = cnt;
+  }
+  // the block's records: triangles [prim0, prim0 + nvalid) of the draw, contiguous from g0
+  __syncthreads();
+  {
+    const u32 prim0 = (blockIdx.x - d.block_base) * 256u;
+    const u32 nvalid = min(256u, d.ntris - prim0);
+    u32 *dst = rec + (u64)(d.tri_base + prim0) * VA_REC_STRIDE;
+    const u32 nwords = nvalid * VA_REC_STRIDE;
+    for(u32 i = threadIdx.x; i < nwords; i += 256u)
+    {
+      const u32 t = i / VA_REC_STRIDE, w = i - t * VA_REC_STRIDE;
+      dst[i] = s_rec[w * 257u + t];
+    }
   }
   // counters: one row of partial sums per workgroup (no contended atomics), columns in
   // VaCounters order from tris_in
@@ -200,19 +227,24 @@ extern "C" __global__ __launch_bounds__(256) void va_setup(
 
 // Pair generation, load-balanced: workgroup b expands primitives [256 b, 256 b + 256). offsets
 // is the exclusive scan of the per-primitive tile counts (total_tris + 1 entries, the last one
-// the pair total); each thread takes pairs of the group by stride and finds its primitive by a
-// binary search over the group's 257 offsets in LDS. Pairs at or beyond cap are not written
-// (the host sees the total and grows the buffers).
+// the pair total); the group's offsets and emit descriptors are staged in LDS, and each thread
+// takes pairs of the group by stride, finding its primitive by a binary search over the 257
+// offsets. Pairs at or beyond cap are not written (the host sees the total and grows the
+// buffers).
 extern "C" __global__ __launch_bounds__(256) void va_emit(
-    const u32 *__restrict__ rec, const u32 *__restrict__ offsets, u32 total_tris, u32 cap,
-    const VaDraw *__restrict__ draws, const VaFrame *__restrict__ frames, u32 tile_shift,
+    const uint2 *__restrict__ emit_info, const u32 *__restrict__ offsets, u32 total_tris, u32 cap,
     u32 *__restrict__ keys, u32 *__restrict__ vals)
 {
   __shared__ u32 s_off[257];
+  __shared__ uint2 s_info[256];
   const u32 t0 = blockIdx.x * 256u;
   s_off[threadIdx.x] = offsets[min(t0 + threadIdx.x, total_tris)];
   if(threadIdx.x == 0u)
     s_off[256] = offsets[min(t0 + 256u, total_tris)];
+  __syncthreads();
+  // descriptors only of primitives that emit (the others were never written)
+  if(t0 + threadIdx.x < total_tris && s_off[threadIdx.x + 1u] != s_off[threadIdx.x])
+    s_info[threadIdx.x] = emit_info[t0 + threadIdx.x];
   __syncthreads();
   const u32 pend = min(s_off[256], cap);
   for(u32 p = s_off[0] + threadIdx.x; p < pend; p += 256u)
@@ -223,30 +255,29 @@ extern "C" __global__ __launch_bounds__(256) void va_emit(
     for(u32 step = 128u; step > 0u; step >>= 1)
       if(s_off[i + step] <= p)
         i += step;
-    const u32 t = t0 + i;
     const u32 k = p - s_off[i];
-    const u32 *r = rec + (u64)t * VA_REC_STRIDE;
-    const u32 bmin = r[VA_R_BBMIN], bmax = r[VA_R_BBMAX];
-    const u32 tx0 = (bmin & 0xffffu) >> tile_shift, tx1 = ((bmax & 0xffffu) - 1u) >> tile_shift;
-    const u32 ty0 = (bmin >> 16) >> tile_shift;
-    const u32 nx = tx1 - tx0 + 1u;
-    const VaFrame &fr = frames[draws[r[VA_R_DRAW]].frame];
-    const u32 tx = tx0 + k % nx, ty = ty0 + k / nx;
-    keys[p] = fr.tile_base + ty * fr.tiles_x + tx;
-    vals[p] = t;
+    const uint2 e = s_info[i];
+    const u32 nx = e.y & 0xffffu, tiles_x = e.y >> 16;
+    const u32 row = k / nx;
+    keys[p] = e.x + row * tiles_x + (k - row * nx);
+    vals[p] = t0 + i;
   }
 }
 
-// Bin ranges from the sorted keys: [start, end) per tile. n = min(*n_dev, cap).
+// Bin bounds from the sorted keys: start[t] = first index whose key is >= t, for every tile
+// t in [0, ntiles] (so tile t's bin is [start[t], start[t + 1])). n = min(*n_dev, cap).
 extern "C" __global__ __launch_bounds__(256) void va_ranges(const u32 *__restrict__ keys, const u32 *__restrict__ n_dev,
-                                                            u32 cap, u32 *__restrict__ start, u32 *__restrict__ end)
+                                                            u32 cap, u32 ntiles, u32 *__restrict__ start)
 {
-  const u32 n = min(*n_dev, cap);
-  const u32 i = blockIdx.x * blockDim.x + threadIdx.x;
-  if(i >= n) return;
-  const u32 k = keys[i];
-  if(i == 0u || keys[i - 1u] != k) start[k] = i;
-  if(i == n - 1u || keys[i + 1u] != k) end[k] = i + 1u;
+  const u32 t = blockIdx.x * blockDim.x + threadIdx.x;
+  if(t > ntiles) return;
+  u32 lo = 0, hi = min(*n_dev, cap);
+  while(lo < hi)
+  {
+    const u32 mid = (lo + hi) >> 1;
+    if(keys[mid] < t) lo = mid + 1u; else hi = mid;
+  }
+  start[t] = lo;
 }
 
 extern "C" __global__ __launch_bounds__(256) void va_fill_u32(u32 *__restrict__ p, u32 n, u32 v)
@@ -421,22 +452,6 @@ VA_INLINE u32 va_wave_exclusive_sum(u32 v, u32 lane)
   return incl - v;
 }
 
-// Position of the n-th (0-based) set bit of m; n < popcount(m).
-VA_INLINE u32 va_nth_set_bit(u64 m, u32 n)
-{
-  u32 pos = 0u, w = (u32)m, c = (u32)__popc(w);
-  if(n >= c) { n -= c; pos = 32u; w = (u32)(m >> 32); }
-  c = (u32)__popc(w & 0xffffu);
-  if(n >= c) { n -= c; pos += 16u; w >>= 16; }
-  c = (u32)__popc(w & 0xffu);
-  if(n >= c) { n -= c; pos += 8u; w >>= 8; }
-  c = (u32)__popc(w & 0xfu);
-  if(n >= c) { n -= c; pos += 4u; w >>= 4; }
-  c = (u32)__popc(w & 0x3u);
-  if(n >= c) { n -= c; pos += 2u; w >>= 2; }
-  return pos + ((n >= (w & 1u)) ? 1u : 0u);
-}
-
 // Coverage only (the integer part of va_bary).
 VA_INLINE bool va_covered(const u32 *__restrict__ r, i32 x, i32 y)
 {
@@ -485,8 +500,7 @@ VA_INLINE void va_raster_tile(const VaDraw *__restrict__ draws, const VaFrame *_
   __shared__ uint4 s_rec4[VA_CHUNK * Q];
   __shared__ u32 s_tri[VA_CHUNK];
   __shared__ u16 s_flags[VA_CHUNK];   // per-record fixed-function state (va_rec_flags)
-  __shared__ u64 s_pass[NW][64];      // per pixel: records of the chunk that passed depth
-  __shared__ u32 s_off[NW][64];       // per pixel: first index in the wave's job order
+  __shared__ u16 s_job[NW][64];       // jobs of one shading round: record << 6 | pixel
   __shared__ float4 s_res[NW][64];    // fragment program results of one shading round
   const u32 *s_rec = (const u32 *)s_rec4;
 
@@ -669,30 +683,30 @@ VA_INLINE void va_raster_tile(const VaDraw *__restrict__ draws, const VaFrame *_
     }
 
     // 4. Shade every passing fragment in rounds of 64, jobs in pixel-major order (pixel p owns
-    //    jobs [off_p, off_p + count_p), in ascending k), then each pixel blends its own results
-    //    of the round in order.
+    //    jobs [off_p, off_p + count_p), in ascending k): each pixel writes its jobs of the round
+    //    to the job list, the wave shades the list 64 wide, then each pixel blends its own
+    //    results of the round in order.
     const u32 cnt = (u32)__popcll(pass);
     const u32 off = va_wave_exclusive_sum(cnt, lane);
     const u32 total = (u32)__builtin_amdgcn_readlane((int)(off + cnt), 63);
-    s_pass[wave][lane] = pass;
-    s_off[wave][lane] = off;
-    u64 rem = pass;
-    u32 next = off;
+    u64 wrem = pass, brem = pass;  // write and blend cursors over this pixel's jobs
+    u32 wnext = off, bnext = off;
+    const u32 mine_end = off + cnt;
     for(u32 base = 0; base < total; base += 64u)
     {
-      va_wave_sync();
-      const u32 j = base + lane;
-      if(j < total)
+      const u32 stop = min(mine_end, base + 64u);
+      // each pixel lists its jobs of this round, in ascending record order
+      for(; wnext < stop; ++wnext)
       {
-        // pixel = last p with off_p <= j (pixels without jobs share the next pixel's offset)
-        // (p + step never exceeds 63: the steps sum to 63)
-        u32 p = 0;
-#pragma unroll
-        for(u32 step = 32u; step > 0u; step >>= 1)
-          if(s_off[wave][p + step] <= j)
-            p += step;
-        // k < n by construction; the clamp keeps a record of this chunk under any mistake
-        const u32 k = min(va_nth_set_bit(s_pass[wave][p], j - s_off[wave][p]), n - 1u);
+        const u32 k = (u32)__builtin_ctzll(wrem);
+        wrem &= wrem - 1ull;
+        s_job[wave][wnext - base] = (u16)((k << 6) | lane);
+      }
+      va_wave_sync();
+      if(base + lane < total)
+      {
+        const u32 job = s_job[wave][lane];
+        const u32 k = job >> 6, p = job & 63u;
         const u32 *r = s_rec + k * VA_REC_STRIDE;
         const i32 x = bx + (i32)(p & 7u), y = by + (i32)(p >> 3);
         float b[3][4];
@@ -707,17 +721,17 @@ VA_INLINE void va_raster_tile(const VaDraw *__restrict__ draws, const VaFrame *_
         s_res[wave][lane] = make_float4(o[0], o[1], o[2], o[3]);
       }
       va_wave_sync();
-      const u32 stop = min(off + cnt, base + 64u);
-      for(; next < stop; ++next)
+      for(; bnext < stop; ++bnext)
       {
-        const u32 k = (u32)__builtin_ctzll(rem);
-        rem &= rem - 1ull;
-        const float4 v = s_res[wave][next - base];
+        const u32 k = (u32)__builtin_ctzll(brem);
+        brem &= brem - 1ull;
+        const float4 v = s_res[wave][bnext - base];
         float o[4] = {v.x, v.y, v.z, v.w};
         va_blend_store_flags(s_flags[k], hdr, o, c0, c1);
       }
+      // the next round rewrites s_job and s_res
+      va_wave_sync();
     }
-    va_wave_sync();
 #endif
   }
 

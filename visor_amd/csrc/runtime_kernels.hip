@@ -95,12 +95,25 @@ __global__ __launch_bounds__(1024) void va_scan_top_kernel(u32 *__restrict__ bsu
   }
 }
 
-// out may alias in: every block reads its whole range into LDS before writing it.
-__global__ __launch_bounds__(256) void va_scan_down_kernel(const u32 *in, u32 n, const u32 *__restrict__ bsum, u32 *out)
+// out may alias in: every block reads its whole range into LDS before writing it. With
+// prefixed != 0, bsum already holds exclusive block offsets; otherwise each block sums the
+// block totals before it itself (cheap for up to a few thousand blocks; saves a launch).
+__global__ __launch_bounds__(256) void va_scan_down_kernel(const u32 *in, u32 n, const u32 *__restrict__ bsum, u32 *out,
+                                                           u32 prefixed)
 {
   __shared__ u32 s_v[RK_SCAN_B];
   __shared__ u32 s_w[4];
   const u32 base = blockIdx.x * RK_SCAN_B;
+  u32 boff;
+  if(prefixed)
+    boff = bsum[blockIdx.x];
+  else
+  {
+    u32 pre = 0;
+    for(u32 i = threadIdx.x; i < blockIdx.x; i += RK_SCAN_T)
+      pre += bsum[i];
+    rk_block_exclusive<4>(pre, s_w, boff);
+  }
 #pragma unroll
   for(u32 i = 0; i < RK_SCAN_I; i++)
   {
@@ -116,7 +129,7 @@ __global__ __launch_bounds__(256) void va_scan_down_kernel(const u32 *in, u32 n,
     t += v[i];
   }
   u32 total;
-  u32 run = bsum[blockIdx.x] + rk_block_exclusive<4>(t, s_w, total);
+  u32 run = boff + rk_block_exclusive<4>(t, s_w, total);
 #pragma unroll
   for(u32 i = 0; i < RK_SCAN_I; i++)
   {
@@ -142,12 +155,13 @@ extern "C" hipError_t va_scan_u32(uint32_t *temp, const uint32_t *in, uint32_t *
   if(n == 0)
     return hipSuccess;
   const u32 nb = scan_blocks(n);
+  const u32 prefixed = nb > 2048u ? 1u : 0u;  // many blocks: scan the block totals separately
   (void)hipGetLastError();  // report this launch's error only
   hipLaunchKernelGGL(va_scan_reduce_kernel, dim3(nb), dim3(RK_SCAN_T), 0, stream, in, n, temp);
-  (void)hipGetLastError();  // report this launch's error only
-  hipLaunchKernelGGL(va_scan_top_kernel, dim3(1), dim3(1024), 0, stream, temp, nb);
-  (void)hipGetLastError();  // report this launch's error only
-  hipLaunchKernelGGL(va_scan_down_kernel, dim3(nb), dim3(RK_SCAN_T), 0, stream, in, n, (const u32 *)temp, out);
+  if(prefixed)
+    hipLaunchKernelGGL(va_scan_top_kernel, dim3(1), dim3(1024), 0, stream, temp, nb);
+  hipLaunchKernelGGL(va_scan_down_kernel, dim3(nb), dim3(RK_SCAN_T), 0, stream, in, n, (const u32 *)temp, out,
+                     prefixed);
   return hipGetLastError();
 }
 
