@@ -168,9 +168,9 @@ def test_offline_pipeline_build(lib, tmp_path, monkeypatch):
 # ---- the device's byte -> float conversion --------------------------------------------------------
 
 def test_unorm8_division_free_is_exact(tmp_path):
-    """va_unorm8 (pipeline_prelude.hip) replaces float(b) / 255.0f by a product and two FMA
-    corrections. Its source is compiled for the host and compared with the IEEE quotient for every
-    byte value."""
+    """va_unorm8 (pipeline_prelude.hip) replaces float(b) / 255.0f by a product and an FMA with a
+    split constant. Its source is compiled for the host and compared with the IEEE quotient for
+    every byte value."""
     src = open(os.path.join(ROOT, "visor_amd", "csrc", "pipeline_prelude.hip")).read()
     m = re.search(r"VA_INLINE float va_unorm8\(u32 b\)\s*\{.*?\n\}", src, flags=re.S)
     assert m

@@ -738,6 +738,11 @@ VkResult Queue::RunPass(const std::vector<Op *> &passes, bool fused_clear, ExecS
   }
   const uint32_t stride = mod->rec_stride;
   const uint32_t nframes = (uint32_t)frames.size(), ndraws = (uint32_t)draws.size();
+  if(ndraws >= (1u << 23))
+  {
+    LogError("a render pass batch holds at most 8388607 draws");
+    return VK_ERROR_FEATURE_NOT_PRESENT;
+  }
   const uint32_t nblocks = (uint32_t)block_draw.size();
   if(!Grow(&d_frames, c_frames, sizeof(VaFrame) * nframes) || !Grow(&d_draws, c_draws, sizeof(VaDraw) * std::max(ndraws, 1u)) ||
      !Grow(&d_rec, c_rec, (size_t)stride * 4 * std::max(tri_total, 1u)) || !Grow(&d_count, c_count, 4ull * (tri_total + 1)) ||

@@ -77,6 +77,20 @@ VA_INLINE void va_block_partials(u64 *row, u64 a, u64 b, u64 c, u64 e)
   }
 }
 
+// Fixed-function state of a record's draw in 9 bits: depth op (0-2), depth write (3), blend
+// enable (4), source and destination colour factor codes (5-6, 7-8: 0 ZERO, 1 ONE and every
+// other factor, 2 SRC_ALPHA, 3 ONE_MINUS_SRC_ALPHA; reference rasterizer.cpp:600-640).
+#define VA_F_DWRITE 8u
+#define VA_F_BLEND 16u
+// record word VA_R_DRAW: draw index in the low VA_DRAW_BITS bits, the 9 flag bits above
+#define VA_DRAW_BITS 23u
+#define VA_DRAW_MASK ((1u << VA_DRAW_BITS) - 1u)
+VA_INLINE u32 va_factor_code(u32 f) { return f == 0u ? 0u : (f == 6u ? 2u : (f == 7u ? 3u : 1u)); }
+VA_INLINE u16 va_rec_flags(const VaDraw &d)
+{
+  return (u16)((d.depth_op & 7u) | (d.depth_write ? VA_F_DWRITE : 0u) | (d.blend_enable ? VA_F_BLEND : 0u) |
+               (va_factor_code(d.src_factor) << 5) | (va_factor_code(d.dst_factor) << 7));
+}
 // reference rasterizer.cpp:100-119 (GetIndex, vertexOffset not applied)
 VA_INLINE u32 va_index(const VaDraw &d, u32 i)
 {
@@ -98,6 +112,7 @@ extern "C" __global__ __launch_bounds__(256) void va_setup(
     uint2 *__restrict__ emit_info, u32 total_tris, u32 *__restrict__ counters)
 {
   __shared__ u32 s_rec[VA_REC_STRIDE * 257];
+  __shared__ u32 s_keep[256];  // 1 for threads whose primitive produced a record
   if(blockIdx.x == 0u)
   {
     if(threadIdx.x < (u32)(sizeof(VaCounters) / 4))
@@ -190,7 +205,7 @@ extern "C" __global__ __launch_bounds__(256) void va_setup(
           r[(VA_R_IW0 + 2) * 257] = va_f2u(iw2);
           r[(VA_R_BBMIN) * 257] = (u32)minx | ((u32)miny << 16);
           r[(VA_R_BBMAX) * 257] = (u32)maxx | ((u32)maxy << 16);
-          r[(VA_R_DRAW) * 257] = di;
+          r[(VA_R_DRAW) * 257] = di | ((u32)va_rec_flags(d) << VA_DRAW_BITS);
 #pragma unroll
           for(int j = 0; j < VA_MAXNV; j++)
           {
@@ -207,7 +222,9 @@ extern "C" __global__ __launch_bounds__(256) void va_setup(
     }
     tile_count[g] = cnt;
   }
-  // the block's records: triangles [prim0, prim0 + nvalid) of the draw, contiguous from g0
+  s_keep[threadIdx.x] = n_tested != 0ull ? 1u : 0u;
+  // the block's records: triangles [prim0, prim0 + nvalid) of the draw, contiguous from g0;
+  // slots of primitives without a record (culled, degenerate, empty bbox) are not written
   __syncthreads();
   {
     const u32 prim0 = (blockIdx.x - d.block_base) * 256u;
@@ -217,7 +234,8 @@ extern "C" __global__ __launch_bounds__(256) void va_setup(
     for(u32 i = threadIdx.x; i < nwords; i += 256u)
     {
       const u32 t = i / VA_REC_STRIDE, w = i - t * VA_REC_STRIDE;
-      dst[i] = s_rec[w * 257u + t];
+      if(s_keep[t])
+        dst[i] = s_rec[w * 257u + t];
     }
   }
   // counters: one row of partial sums per workgroup (no contended atomics), columns in
@@ -326,17 +344,6 @@ VA_INLINE bool va_bary(const u32 *__restrict__ r, i32 x, i32 y, float b[4], floa
   return covered;
 }
 
-// Fixed-function state of a record's draw in 9 bits: depth op (0-2), depth write (3), blend
-// enable (4), source and destination colour factor codes (5-6, 7-8: 0 ZERO, 1 ONE and every
-// other factor, 2 SRC_ALPHA, 3 ONE_MINUS_SRC_ALPHA; reference rasterizer.cpp:600-640).
-#define VA_F_DWRITE 8u
-#define VA_F_BLEND 16u
-VA_INLINE u32 va_factor_code(u32 f) { return f == 0u ? 0u : (f == 6u ? 2u : (f == 7u ? 3u : 1u)); }
-VA_INLINE u16 va_rec_flags(const VaDraw &d)
-{
-  return (u16)((d.depth_op & 7u) | (d.depth_write ? VA_F_DWRITE : 0u) | (d.blend_enable ? VA_F_BLEND : 0u) |
-               (va_factor_code(d.src_factor) << 5) | (va_factor_code(d.dst_factor) << 7));
-}
 VA_INLINE float va_factor_value(u32 code, float a)
 {
   return code == 0u ? 0.0f : (code == 2u ? a : (code == 3u ? 1.0f - a : 1.0f));
@@ -570,7 +577,7 @@ VA_INLINE void va_raster_tile(const VaDraw *__restrict__ draws, const VaFrame *_
     }
     __syncthreads();
     for(u32 k = threadIdx.x; k < n; k += NT)
-      s_flags[k] = va_rec_flags(draws[s_rec[k * VA_REC_STRIDE + VA_R_DRAW]]);
+      s_flags[k] = (u16)(s_rec[k * VA_REC_STRIDE + VA_R_DRAW] >> VA_DRAW_BITS);
     __syncthreads();
 
 #if VA_FS_KILL
@@ -587,7 +594,7 @@ VA_INLINE void va_raster_tile(const VaDraw *__restrict__ draws, const VaFrame *_
       bool cov = va_bary(r, px, py, b[0], &pixdepth);
       cov = cov && inside && px >= minx && px < maxx && py >= miny && py < maxy;
       if(__ballot(cov) == 0ull || !cov) continue;
-      const VaDraw &d = draws[r[VA_R_DRAW]];
+      const VaDraw &d = draws[r[VA_R_DRAW] & VA_DRAW_MASK];
       n_cov++;
       if(d.depth_op != 7u && has_depth && !va_depth_pass(d.depth_op, pixdepth, dep)) continue;
       va_bary(r, px + 1, py, b[1], nullptr);
@@ -715,7 +722,7 @@ VA_INLINE void va_raster_tile(const VaDraw *__restrict__ draws, const VaFrame *_
         va_bary(r, x + 1, y, b[1], nullptr);
         va_bary(r, x, y + 1, b[2], nullptr);
 #endif
-        const VaDraw &d = draws[r[VA_R_DRAW]];
+        const VaDraw &d = draws[r[VA_R_DRAW] & VA_DRAW_MASK];
         float o[4];
         va_fs(d.slot, d, (const float *)(r + VA_REC_HDR), b, o);
         s_res[wave][lane] = make_float4(o[0], o[1], o[2], o[3]);

@@ -17,17 +17,16 @@ VA_INLINE float va_u2f(u32 u) { return __uint_as_float(u); }
 VA_INLINE u32 va_f2u(float f) { return __float_as_uint(f); }
 VA_INLINE i32 va_f2i(float f) { return (i32)f; }
 
-// byte / 255.0f, correctly rounded, without a division: q = b * RN(1/255), then one Newton
-// correction with exact FMAs. Equal to the IEEE quotient for every b in 0..255 (checked
-// exhaustively by the test suite), so texel and attribute values match the reference's
-// float(byte) / 255.0f bit for bit.
+// byte / 255.0f, correctly rounded, without a division: 1/255 split into hi + lo (hi = RN(1/255),
+// lo = RN(1/255 - hi)), then one product and one FMA, b * hi + RN(b * lo), rounded once. Equal to
+// the IEEE quotient for every b in 0..255 (checked exhaustively by the test suite), so texel and
+// attribute values match the reference's float(byte) / 255.0f bit for bit.
 VA_INLINE float va_unorm8(u32 b)
 {
-  const float c = 0.0039215688593685627f;  // RN(1/255)
+  const float hi = 0.0039215688593685627f;  // RN(1/255)
+  const float lo = -0x1.fdfdfep-33f;        // RN(1/255 - hi)
   const float x = (float)b;
-  const float q = x * c;
-  const float r = __builtin_fmaf(-q, 255.0f, x);
-  return __builtin_fmaf(r, c, q);
+  return __builtin_fmaf(x, hi, x * lo);
 }
 
 // ---------------------------------------------------------------------------------------------
