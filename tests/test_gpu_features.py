@@ -1,0 +1,160 @@
+"""Fixed-function and sampler features on the GPU, each against the oracle: every depth compare op
+with and without depth writes, every cull mode x front face, all blend-factor pairs the reference
+implements, triangle strips, 16-bit indices, a loading (not clearing) second render pass, clear
+values outside [0, 1], and sampler modes. Small targets; bit-exact colour and depth."""
+import dataclasses
+
+import numpy as np
+import pytest
+
+from oracle import refrast
+from visor_amd import render, scenes
+from visor_amd.vk import (ADDR_CLAMP, ADDR_REPEAT, BF, CMP, CULL_BACK, CULL_FRONT, CULL_NONE, FILTER_LINEAR,
+                          FILTER_NEAREST, FRONT_CCW, FRONT_CW, INDEX_U16, MIPMAP_LINEAR, MIPMAP_NEAREST, TOPO_STRIP)
+
+pytestmark = pytest.mark.gpu
+
+
+def _run(ctx, s, passes=None, batched=True):
+    r = render.SceneRenderer(ctx, s)
+    ctx.set_mode(batched=batched, tile=8)
+    if passes is not None:
+        r.cb = r.record(passes=passes)
+    r.render()
+    return r.read_color(), (r.read_depth() if r.has_depth else None), ctx.stats(0)
+
+
+def _check(ctx, s, ref=None, passes=None, batched=True):
+    ref = ref if ref is not None else refrast.render(s)
+    col, dep, st = _run(ctx, s, passes, batched)
+    bad = np.argwhere((col != ref["color"]).any(-1))
+    assert bad.size == 0, "colour differs at %d pixels, first %s gpu %s ref %s" % (
+        len(bad), bad[:2].tolist(), col[tuple(bad[0])].tolist(), ref["color"][tuple(bad[0])].tolist())
+    if dep is not None:
+        bad = np.argwhere(dep.view(np.uint32) != ref["depth"].view(np.uint32))
+        assert bad.size == 0, "depth differs at %d pixels" % len(bad)
+    if passes is None:
+        assert st["pixels_written"] == ref["counters"]["pixels_written"]
+        assert st["depth_passed"] == ref["counters"]["depth_passed"]
+    return ref
+
+
+def _soup(ntris=300, w=96, h=80, seed=5, **pipe):
+    s = scenes.config1(ntris, w, h, seed=seed)
+    s.pipelines = {"main": dataclasses.replace(s.pipelines["main"], **pipe)}
+    return s
+
+
+@pytest.mark.parametrize("op", sorted(CMP))
+@pytest.mark.parametrize("write", [True, False])
+def test_depth_compare_ops(ctx, op, write):
+    s = _soup(depth_op=CMP[op], depth_write=write)
+    s.clear_depth = 0.5
+    ref = _check(ctx, s)
+    if op not in ("NEVER", "EQUAL"):
+        assert ref["counters"]["depth_passed"] > 0
+
+
+def test_equal_redraw_passes_where_depth_was_written(ctx):
+    """A LESS pass writes depth; a loading pass redraws the same triangles with EQUAL and other
+    colours: exactly the frontmost fragments pass again."""
+    s = _soup(ntris=200, w=80, h=64, seed=14)
+    s.pipelines["eq"] = dataclasses.replace(s.pipelines["main"], depth_op=CMP["EQUAL"], depth_write=False,
+                                            blend=True, src=BF["ONE"], dst=BF["ONE"])
+    d = s.draws[0]
+    first = scenes.Draw("main", d.count, 0, False, d.push)
+    again = scenes.Draw("eq", d.count, 0, False, d.push)
+    s.draws = [first, again]
+    ref = refrast.render(s)
+    assert ref["counters"]["depth_passed"] > 0
+    _check(ctx, s, ref=ref, passes=[([first], True), ([again], False)])
+
+
+@pytest.mark.parametrize("cull", [CULL_NONE, CULL_FRONT, CULL_BACK])
+@pytest.mark.parametrize("front", [FRONT_CCW, FRONT_CW])
+def test_cull_and_front_face(ctx, cull, front):
+    _check(ctx, _soup(cull=cull, front=front))
+
+
+FACTORS = ["ZERO", "ONE", "SRC_ALPHA", "ONE_MINUS_SRC_ALPHA"]
+
+
+@pytest.mark.parametrize("src", FACTORS)
+@pytest.mark.parametrize("dst", FACTORS)
+def test_blend_factor_pairs(ctx, src, dst):
+    s = _soup(ntris=200, blend=True, src=BF[src], dst=BF[dst], depth_write=False)
+    s.clear_color = (0.2, 0.6, 0.9, 0.5)
+    _check(ctx, s)
+
+
+def test_triangle_strip(ctx):
+    rng = scenes.Rng(8)
+    n = 240
+    pos = np.zeros((n, 4), np.float32)
+    pos[:, 0] = np.linspace(-1.1, 1.1, n).astype(np.float32) + rng.uniform(n, -0.05, 0.05)
+    pos[:, 1] = np.where(np.arange(n) % 2 == 0, -0.6, 0.6).astype(np.float32) + rng.uniform(n, -0.3, 0.3)
+    pos[:, 2] = rng.uniform(n, 0.0, 1.0)
+    pos[:, 3] = 1.0
+    col = rng.bytes(n * 4).reshape(n, 4)
+    pipes = {"main": scenes.Pipeline("vs_mvp_pos4_col", "fs_color", topology=TOPO_STRIP, cull=CULL_BACK)}
+    s = scenes.Scene("strip", 128, 96, scenes.interleave_pos_col(pos, col), 20, scenes.VATTRS_POS_COL, pipes,
+                     [scenes.Draw("main", n, 0, False, scenes.push_block(scenes.IDENTITY))])
+    ref = _check(ctx, s)
+    assert ref["counters"]["tris_out"] < ref["counters"]["tris_in"] == n - 2
+
+
+@pytest.mark.parametrize("batched", [True, False])
+def test_indexed_u16(ctx, batched):
+    rng = scenes.Rng(9)
+    nv, nt = 400, 350
+    pos = np.zeros((nv, 4), np.float32)
+    pos[:, 0] = rng.uniform(nv, -1.0, 1.0)
+    pos[:, 1] = rng.uniform(nv, -1.0, 1.0)
+    pos[:, 2] = rng.uniform(nv, 0.0, 1.0)
+    pos[:, 3] = 1.0
+    col = rng.bytes(nv * 4).reshape(nv, 4)
+    idx = (rng.raw(3 * nt) % np.uint64(nv)).astype(np.uint16)
+    pipes = {"main": scenes.Pipeline("vs_mvp_pos4_col", "fs_color")}
+    draws = [scenes.Draw("main", 3 * 200, 0, True, scenes.push_block(scenes.IDENTITY)),
+             scenes.Draw("main", 3 * 150, 3 * 200, True, scenes.push_block(scenes.IDENTITY))]
+    s = scenes.Scene("u16", 80, 112, scenes.interleave_pos_col(pos, col), 20, scenes.VATTRS_POS_COL, pipes, draws,
+                     index_data=idx.tobytes(), index_type=INDEX_U16)
+    _check(ctx, s, batched=batched)
+
+
+@pytest.mark.parametrize("batched", [True, False])
+def test_second_pass_loads_previous_contents(ctx, batched):
+    """Clear + first half of the draws, then a LOAD pass with the second half: equals one pass
+    with all draws."""
+    s = _soup(ntris=400, w=72, h=72, seed=12)
+    n = s.draws[0].count
+    half = (n // 6) * 3
+    a = scenes.Draw("main", half, 0, False, s.draws[0].push)
+    b = scenes.Draw("main", n - half, half, False, s.draws[0].push)
+    ref = refrast.render(s)
+    _check(ctx, s, ref=ref, passes=[([a], True), ([b], False)], batched=batched)
+
+
+def test_clear_values_outside_unit_range(ctx):
+    """Clears store trunc(v * 255) without clamping (reference rasterizer.cpp:332-361)."""
+    s = _soup(ntris=40, w=48, h=40)
+    s.clear_color = (1.25, -0.25, 0.5, 2.0)
+    s.clear_depth = 0.75
+    _check(ctx, s)
+
+
+SAMPLERS = [
+    dict(mag_filter=FILTER_NEAREST, min_filter=FILTER_NEAREST, mipmap_mode=MIPMAP_NEAREST, address=ADDR_REPEAT),
+    dict(mag_filter=FILTER_LINEAR, min_filter=FILTER_NEAREST, mipmap_mode=MIPMAP_LINEAR, address=ADDR_CLAMP),
+    dict(mag_filter=FILTER_NEAREST, min_filter=FILTER_LINEAR, mipmap_mode=MIPMAP_NEAREST, address=ADDR_CLAMP,
+         lod_bias=1.0),
+    dict(mag_filter=FILTER_LINEAR, min_filter=FILTER_LINEAR, mipmap_mode=MIPMAP_LINEAR, address=ADDR_REPEAT,
+         lod_bias=-1.5, min_lod=0.5),
+]
+
+
+@pytest.mark.parametrize("k", range(len(SAMPLERS)))
+def test_sampler_modes(ctx, k):
+    s = scenes.config3(ndraws=10, width=160, height=120, tex_size=64, max_tris=1024, seed=21)
+    s.textures = [dataclasses.replace(t, **SAMPLERS[k]) for t in s.textures]
+    _check(ctx, s)

@@ -188,18 +188,23 @@ class SceneRenderer:
             self.depth, self.dmem, self.dview, self.dsize = ctx.image(s.width, s.height, vk.FMT["D32_SFLOAT"],
                                                                      host_visible=host_visible_targets)
             views.append(self.dview)
-        atts = [vk.AttachmentDescription(format=s.color_format, samples=1, loadOp=vk.LOAD_OP_CLEAR)]
-        if self.has_depth:
-            atts.append(vk.AttachmentDescription(format=vk.FMT["D32_SFLOAT"], samples=1, loadOp=vk.LOAD_OP_CLEAR))
-        att_arr = (vk.AttachmentDescription * len(atts))(*atts)
-        cref = vk.AttachmentReference(attachment=0, layout=2)
-        dref = vk.AttachmentReference(attachment=1, layout=3)
-        sub = vk.SubpassDescription(colorAttachmentCount=1, pColorAttachments=C.pointer(cref),
-                                    pDepthStencilAttachment=C.pointer(dref) if self.has_depth else None)
-        rpci = vk.RenderPassCreateInfo(attachmentCount=len(atts), pAttachments=att_arr, subpassCount=1,
-                                       pSubpasses=C.pointer(sub))
-        self.rp = Handle()
-        i.check(i.vkCreateRenderPass(dev, C.byref(rpci), None, C.byref(self.rp)), "vkCreateRenderPass")
+        def render_pass(load_op):
+            atts = [vk.AttachmentDescription(format=s.color_format, samples=1, loadOp=load_op)]
+            if self.has_depth:
+                atts.append(vk.AttachmentDescription(format=vk.FMT["D32_SFLOAT"], samples=1, loadOp=load_op))
+            att_arr = (vk.AttachmentDescription * len(atts))(*atts)
+            cref = vk.AttachmentReference(attachment=0, layout=2)
+            dref = vk.AttachmentReference(attachment=1, layout=3)
+            sub = vk.SubpassDescription(colorAttachmentCount=1, pColorAttachments=C.pointer(cref),
+                                        pDepthStencilAttachment=C.pointer(dref) if self.has_depth else None)
+            rpci = vk.RenderPassCreateInfo(attachmentCount=len(atts), pAttachments=att_arr, subpassCount=1,
+                                           pSubpasses=C.pointer(sub))
+            rp = Handle()
+            i.check(i.vkCreateRenderPass(dev, C.byref(rpci), None, C.byref(rp)), "vkCreateRenderPass")
+            return rp
+
+        self.rp = render_pass(vk.LOAD_OP_CLEAR)
+        self.rp_load = render_pass(vk.LOAD_OP_LOAD)  # compatible: same formats, keeps the contents
         varr = (Handle * len(views))(*views)
         fci = vk.FramebufferCreateInfo(renderPass=self.rp, attachmentCount=len(views), pAttachments=varr,
                                        width=s.width, height=s.height, layers=1)
@@ -275,15 +280,24 @@ class SceneRenderer:
         i.check(i.vkCreateGraphicsPipelines(self.ctx.device, None, 1, C.byref(gci), None, C.byref(pl)), "vkCreateGraphicsPipelines")
         return pl
 
-    def record(self, draws=None):
-        """Records one render pass with the scene's draws (or a subset) and returns the buffer."""
-        ctx, i, s = self.ctx, self.ctx.icd, self.scene
+    def record(self, draws=None, passes=None):
+        """Records one render pass with the scene's draws (or a subset) and returns the buffer.
+        passes: a list of (draws, clear) recorded as consecutive render passes instead, the ones
+        with clear False loading the previous contents."""
+        ctx, i = self.ctx, self.ctx.icd
         cb = ctx.begin()
+        for pd, clear in (passes if passes is not None else [(self.scene.draws if draws is None else draws, True)]):
+            self._record_pass(cb, pd, clear)
+        i.check(i.vkEndCommandBuffer(cb), "vkEndCommandBuffer")
+        return cb
+
+    def _record_pass(self, cb, draws, clear):
+        i, s = self.ctx.icd, self.scene
         clears = (vk.ClearValue * 2)()
         for k in range(4):
             clears[0].color[k] = s.clear_color[k]
         clears[1].depthStencil[0] = s.clear_depth if s.clear_depth is not None else 1.0
-        rbi = vk.RenderPassBeginInfo(renderPass=self.rp, framebuffer=self.fb,
+        rbi = vk.RenderPassBeginInfo(renderPass=self.rp if clear else self.rp_load, framebuffer=self.fb,
                                      renderArea=vk.Rect2D(extent=vk.Extent2D(width=s.width, height=s.height)),
                                      clearValueCount=2 if self.has_depth else 1, pClearValues=clears)
         i.vkCmdBeginRenderPass(cb, C.byref(rbi), 0)
@@ -293,7 +307,7 @@ class SceneRenderer:
         if self.ib is not None:
             i.vkCmdBindIndexBuffer(cb, self.ib, 0, s.index_type)
         cur_pipe, cur_set = None, None
-        for d in (s.draws if draws is None else draws):
+        for d in draws:
             if d.pipeline != cur_pipe:
                 i.vkCmdBindPipeline(cb, 0, self.pipelines[d.pipeline])
                 cur_pipe = d.pipeline
@@ -308,8 +322,6 @@ class SceneRenderer:
             else:
                 i.vkCmdDraw(cb, d.count, 1, d.first, 0)
         i.vkCmdEndRenderPass(cb)
-        i.check(i.vkEndCommandBuffer(cb), "vkEndCommandBuffer")
-        return cb
 
     def render(self, draws=None):
         """Records, submits and waits; returns the wall time of the submit in seconds."""
