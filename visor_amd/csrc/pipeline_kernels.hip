@@ -102,8 +102,9 @@ VA_INLINE u32 va_index(const VaDraw &d, u32 i)
 
 // One workgroup covers 256 primitives of a single draw (block_draw maps workgroup -> draw), so
 // the draw's state (push constants, vertex layout, descriptors) is read with scalar loads.
-// Records are assembled in LDS (word w of the thread's record at s_rec[w * 257 + thread]: no
-// bank conflicts) and written out as the block's contiguous record range with coalesced stores.
+// Records are assembled in registers, staged in LDS as 16-byte groups (group q of thread t at
+// s_rec4[q * 257 + t], padded against bank conflicts) and written out as the block's contiguous
+// record range with coalesced 16-byte stores.
 // emit_info[g] = (first tile key, tiles per bbox row | frame tiles per row << 16) for va_emit.
 // Workgroup 0 also zeroes the frame counters and the count-scan's last element.
 extern "C" __global__ __launch_bounds__(256) void va_setup(
@@ -111,8 +112,9 @@ extern "C" __global__ __launch_bounds__(256) void va_setup(
     u32 tile_shift, u32 *__restrict__ rec, u32 *__restrict__ tile_count, u64 *__restrict__ part,
     uint2 *__restrict__ emit_info, u32 total_tris, u32 *__restrict__ counters)
 {
-  __shared__ u32 s_rec[VA_REC_STRIDE * 257];
-  __shared__ u32 s_keep[256];  // 1 for threads whose primitive produced a record
+  constexpr u32 Q = VA_REC_STRIDE / 4u;
+  __shared__ uint4 s_rec4[Q * 257];  // 16-byte group q of thread t's record at [q * 257 + t]
+  __shared__ u32 s_keep[256];        // 1 for threads whose primitive produced a record
   if(blockIdx.x == 0u)
   {
     if(threadIdx.x < (u32)(sizeof(VaCounters) / 4))
@@ -185,34 +187,40 @@ extern "C" __global__ __launch_bounds__(256) void va_setup(
         if(maxx > minx && maxy > miny)
         {
           n_tested = (u64)(maxx - minx) * (u64)(maxy - miny);
-          u32 *r = s_rec + threadIdx.x;  // word w at r[w * 257]
+          u32 r[VA_REC_STRIDE];
+#pragma unroll
+          for(u32 w = 0; w < VA_REC_STRIDE; w++)
+            r[w] = 0u;
           const u32 bm = (u32)barymul;
-          r[(VA_R_X0) * 257] = (u32)X[0];
-          r[(VA_R_Y0) * 257] = (u32)Y[0];
+          r[VA_R_X0] = (u32)X[0];
+          r[VA_R_Y0] = (u32)Y[0];
           // barymul folded into the edge coefficients: exact in wrapping arithmetic
-          r[(VA_R_ABX) * 257] = (u32)(X[1] - X[0]) * bm;
-          r[(VA_R_ABY) * 257] = (u32)(Y[1] - Y[0]) * bm;
-          r[(VA_R_ACX) * 257] = (u32)(X[2] - X[0]) * bm;
-          r[(VA_R_ACY) * 257] = (u32)(Y[2] - Y[0]) * bm;
-          r[(VA_R_AREA) * 257] = (u32)flipped;
-          r[(VA_R_INVAREA) * 257] = va_f2u(1.0f / (float)flipped);
+          r[VA_R_ABX] = (u32)(X[1] - X[0]) * bm;
+          r[VA_R_ABY] = (u32)(Y[1] - Y[0]) * bm;
+          r[VA_R_ACX] = (u32)(X[2] - X[0]) * bm;
+          r[VA_R_ACY] = (u32)(Y[2] - Y[0]) * bm;
+          r[VA_R_AREA] = (u32)flipped;
+          r[VA_R_INVAREA] = va_f2u(1.0f / (float)flipped);
           const float iw0 = 1.0f / pos[0][3], iw1 = 1.0f / pos[1][3], iw2 = 1.0f / pos[2][3];
-          r[(VA_R_Z0 + 0) * 257] = va_f2u(pos[0][2] * iw0);
-          r[(VA_R_Z0 + 1) * 257] = va_f2u(pos[1][2] * iw1);
-          r[(VA_R_Z0 + 2) * 257] = va_f2u(pos[2][2] * iw2);
-          r[(VA_R_IW0 + 0) * 257] = va_f2u(iw0);
-          r[(VA_R_IW0 + 1) * 257] = va_f2u(iw1);
-          r[(VA_R_IW0 + 2) * 257] = va_f2u(iw2);
-          r[(VA_R_BBMIN) * 257] = (u32)minx | ((u32)miny << 16);
-          r[(VA_R_BBMAX) * 257] = (u32)maxx | ((u32)maxy << 16);
-          r[(VA_R_DRAW) * 257] = di | ((u32)va_rec_flags(d) << VA_DRAW_BITS);
+          r[VA_R_Z0 + 0] = va_f2u(pos[0][2] * iw0);
+          r[VA_R_Z0 + 1] = va_f2u(pos[1][2] * iw1);
+          r[VA_R_Z0 + 2] = va_f2u(pos[2][2] * iw2);
+          r[VA_R_IW0 + 0] = va_f2u(iw0);
+          r[VA_R_IW0 + 1] = va_f2u(iw1);
+          r[VA_R_IW0 + 2] = va_f2u(iw2);
+          r[VA_R_BBMIN] = (u32)minx | ((u32)miny << 16);
+          r[VA_R_BBMAX] = (u32)maxx | ((u32)maxy << 16);
+          r[VA_R_DRAW] = di | ((u32)va_rec_flags(d) << VA_DRAW_BITS);
 #pragma unroll
           for(int j = 0; j < VA_MAXNV; j++)
           {
-            r[(VA_REC_HDR + 3 * j + 0) * 257] = va_f2u(vary[0][j]);
-            r[(VA_REC_HDR + 3 * j + 1) * 257] = va_f2u(vary[1][j]);
-            r[(VA_REC_HDR + 3 * j + 2) * 257] = va_f2u(vary[2][j]);
+            r[VA_REC_HDR + 3 * j + 0] = va_f2u(vary[0][j]);
+            r[VA_REC_HDR + 3 * j + 1] = va_f2u(vary[1][j]);
+            r[VA_REC_HDR + 3 * j + 2] = va_f2u(vary[2][j]);
           }
+#pragma unroll
+          for(u32 q = 0; q < Q; q++)
+            s_rec4[q * 257u + threadIdx.x] = make_uint4(r[4 * q], r[4 * q + 1], r[4 * q + 2], r[4 * q + 3]);
           const u32 tx0 = (u32)minx >> tile_shift, tx1 = (u32)(maxx - 1) >> tile_shift;
           const u32 ty0 = (u32)miny >> tile_shift, ty1 = (u32)(maxy - 1) >> tile_shift;
           cnt = (tx1 - tx0 + 1u) * (ty1 - ty0 + 1u);
@@ -229,13 +237,13 @@ extern "C" __global__ __launch_bounds__(256) void va_setup(
   {
     const u32 prim0 = (blockIdx.x - d.block_base) * 256u;
     const u32 nvalid = min(256u, d.ntris - prim0);
-    u32 *dst = rec + (u64)(d.tri_base + prim0) * VA_REC_STRIDE;
-    const u32 nwords = nvalid * VA_REC_STRIDE;
-    for(u32 i = threadIdx.x; i < nwords; i += 256u)
+    uint4 *dst = (uint4 *)(rec + (u64)(d.tri_base + prim0) * VA_REC_STRIDE);
+    const u32 nq = nvalid * Q;
+    for(u32 i = threadIdx.x; i < nq; i += 256u)
     {
-      const u32 t = i / VA_REC_STRIDE, w = i - t * VA_REC_STRIDE;
+      const u32 t = i / Q, q = i - t * Q;
       if(s_keep[t])
-        dst[i] = s_rec[w * 257u + t];
+        dst[i] = s_rec4[q * 257u + t];
     }
   }
   // counters: one row of partial sums per workgroup (no contended atomics), columns in
