@@ -123,6 +123,9 @@ bool Device::Init(std::string &err)
   }
   std::string name = props.gcnArchName;
   arch = name.substr(0, name.find(':'));
+  int khz = 0;
+  if(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, 0) == hipSuccess && khz > 0)
+    wall_khz = khz;
   initialized = true;
   return true;
 }

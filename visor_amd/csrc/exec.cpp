@@ -90,7 +90,7 @@ Queue::~Queue()
     worker.join();
   }
   void *bufs[] = {d_frames, d_draws, d_rec, d_count, d_offs, d_keys, d_vals, d_keys2, d_vals2, d_start, d_end, d_ctr, d_temp,
-                  d_blocks, d_part, d_stemp, d_einfo};
+                  d_blocks, d_part, d_stemp, d_einfo, d_stamps};
   for(void *b : bufs)
     if(b)
       hipFree(b);
@@ -747,7 +747,7 @@ VkResult Queue::RunPass(const std::vector<Op *> &passes, bool fused_clear, ExecS
   if(!Grow(&d_frames, c_frames, sizeof(VaFrame) * nframes) || !Grow(&d_draws, c_draws, sizeof(VaDraw) * std::max(ndraws, 1u)) ||
      !Grow(&d_rec, c_rec, (size_t)stride * 4 * std::max(tri_total, 1u)) || !Grow(&d_count, c_count, 4ull * (tri_total + 1)) ||
      !Grow(&d_offs, c_offs, 4ull * (tri_total + 1)) || !Grow(&d_start, c_start, 4ull * (tile_total + 1)) ||
-     !Grow(&d_einfo, c_einfo, 8ull * std::max(tri_total, 1u)) ||
+     !Grow(&d_einfo, c_einfo, 8ull * std::max(tri_total, 1u)) || !Grow(&d_stamps, c_stamps, 64) ||
      !Grow(&d_ctr, c_ctr, sizeof(VaCounters)) ||
      !Grow(&d_blocks, c_blocks, 4ull * std::max<size_t>(nblocks, 1)) ||
      !Grow(&d_part, c_part, 32ull * (std::max<size_t>(nblocks, tile_total) + 1)) ||
@@ -816,8 +816,10 @@ VkResult Queue::RunPass(const std::vector<Op *> &passes, bool fused_clear, ExecS
     if((e = (x)) != hipSuccess) \
       return e;                 \
   } while(0)
+      uint64_t *stamps = (uint64_t *)d_stamps;
       if(stage == 0)
       {
+        VA_TRY(va_stamp_async(stamps, stream));
         if(tri_total)
         {
           // the setup kernel zeroes the counters and the scan's last count itself
@@ -836,6 +838,7 @@ VkResult Queue::RunPass(const std::vector<Op *> &passes, bool fused_clear, ExecS
       }
       else if(stage == 1)
       {
+        VA_TRY(va_stamp_async(stamps + 1, stream));
         if(tri_total)
         {
           uint32_t *keys = (uint32_t *)d_keys, *vals = (uint32_t *)d_vals;
@@ -884,7 +887,8 @@ VkResult Queue::RunPass(const std::vector<Op *> &passes, bool fused_clear, ExecS
       key.dims[5] = cap;
       key.dims[6] = fused_clear ? 1u : 0u;
       void *bufs[] = {d_frames, d_draws, d_rec, d_count, d_offs, d_keys, d_vals, d_keys2, d_vals2,
-                      d_stemp, d_temp, d_start, d_einfo, d_ctr, d_blocks, d_part};
+                      d_stemp, d_temp, d_start, d_einfo, d_ctr, d_blocks, d_part, d_stamps};
+      static_assert(sizeof(bufs) <= sizeof(key.bufs), "graph key holds every buffer");
       memcpy(key.bufs, bufs, sizeof(bufs));
       for(size_t i = 0; i < graph_cache.size(); i++)
         if(memcmp(&graph_cache[i].key, &key, sizeof(key)) == 0)
@@ -898,15 +902,19 @@ VkResult Queue::RunPass(const std::vector<Op *> &passes, bool fused_clear, ExecS
         GraphSet set;
         set.key = key;
         hipError_t err = hipSuccess;
-        for(int stage = 0; stage < 3 && err == hipSuccess; stage++)
+        for(int part = 0; part < 2 && err == hipSuccess; part++)
         {
           hipError_t e = hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal);
-          hipError_t er = e == hipSuccess ? enqueue(stage) : e;
+          hipError_t er = e;
+          if(e == hipSuccess)
+            er = part == 0 ? enqueue(0) : enqueue(2);
+          if(e == hipSuccess && er == hipSuccess && part == 0)
+            er = enqueue(1);
           hipGraph_t g = nullptr;
           hipError_t ec = e == hipSuccess ? hipStreamEndCapture(stream, &g) : e;
           err = er != hipSuccess ? er : ec;
           if(err == hipSuccess && g)
-            err = hipGraphInstantiate(&set.exec[stage], g, nullptr, nullptr, 0);
+            err = hipGraphInstantiate(&set.exec[part], g, nullptr, nullptr, 0);
           else if(err == hipSuccess)
             err = hipErrorUnknown;
           if(g)
@@ -934,18 +942,26 @@ VkResult Queue::RunPass(const std::vector<Op *> &passes, bool fused_clear, ExecS
         }
       }
     }
-    for(int stage = 0; stage < 3; stage++)
+    // setup and binning as one launch (one graph), then the raster stage, with HIP events
+    // around both; the setup / binning split comes from the device stamps
+    HIPCHK(hipEventRecord(ev[0], stream));
+    if(gs)
+      HIPCHK(hipGraphLaunch(gs->exec[0], stream));
+    else
     {
-      HIPCHK(hipEventRecord(ev[stage], stream));
-      if(gs)
-        HIPCHK(hipGraphLaunch(gs->exec[stage], stream));
-      else
-        HIPCHK(enqueue(stage));
+      HIPCHK(enqueue(0));
+      HIPCHK(enqueue(1));
     }
+    HIPCHK(hipEventRecord(ev[2], stream));
+    if(gs)
+      HIPCHK(hipGraphLaunch(gs->exec[1], stream));
+    else
+      HIPCHK(enqueue(2));
     HIPCHK(hipEventRecord(ev[3], stream));
     if(gs)
       st.graph_launches++;
     HIPCHK(hipMemcpyAsync(h_pinned + 64, d_ctr, sizeof(VaCounters), hipMemcpyDeviceToHost, stream));
+    HIPCHK(hipMemcpyAsync(h_pinned + 128, d_stamps, 16, hipMemcpyDeviceToHost, stream));
     HIPCHK(hipMemcpyAsync(h_pinned, total_dev, 4, hipMemcpyDeviceToHost, stream));
     HIPCHK(hipStreamSynchronize(stream));
     total_pairs = h_pinned[0];
@@ -963,22 +979,26 @@ VkResult Queue::RunPass(const std::vector<Op *> &passes, bool fused_clear, ExecS
   c.draws = ndraws;
   st.last_cfg = cfg;
   {
-    float t01 = 0.0f, t12 = 0.0f, t23 = 0.0f;
-    hipError_t e1 = hipEventElapsedTime(&t01, ev[0], ev[1]);
-    hipError_t e2 = hipEventElapsedTime(&t12, ev[1], ev[2]);
-    hipError_t e3 = hipEventElapsedTime(&t23, ev[2], ev[3]);
-    if(e1 != hipSuccess || e2 != hipSuccess || e3 != hipSuccess)
+    float t02 = 0.0f, t23 = 0.0f;
+    hipError_t e1 = hipEventElapsedTime(&t02, ev[0], ev[2]);
+    hipError_t e2 = hipEventElapsedTime(&t23, ev[2], ev[3]);
+    if(e1 != hipSuccess || e2 != hipSuccess)
     {
       (void)hipGetLastError();
-      LogError("stage timing unavailable: %s", hipGetErrorString(e1 != hipSuccess ? e1 : (e2 != hipSuccess ? e2 : e3)));
+      LogError("stage timing unavailable: %s", hipGetErrorString(e1 != hipSuccess ? e1 : e2));
     }
     else
     {
-      st.setup_ms += t01;
-      st.bin_ms += t12;
+      uint64_t s0, s1;
+      memcpy(&s0, h_pinned + 128, 8);
+      memcpy(&s1, h_pinned + 130, 8);
+      double setup = s1 > s0 ? (double)(s1 - s0) / (double)Device::Get().wall_khz : 0.0;
+      setup = std::min(setup, (double)t02);
+      st.setup_ms += setup;
+      st.bin_ms += t02 - setup;
       st.raster_ms += t23;
       if(tuning)
-        RecordTuning(sig, cfg, (double)t01 + t12 + t23);
+        RecordTuning(sig, cfg, (double)t02 + t23);
     }
   }
 

@@ -356,6 +356,7 @@ public:
   bool ok() const { return initialized; }
   int device_id = 0;
   hipDeviceProp_t props{};
+  int wall_khz = 100000;  // device wall clock rate (wall_clock64 ticks per millisecond)
   std::string arch = "gfx950";
 
   std::shared_ptr<ShaderPair> MakePair(VkShaderModule_T *vs, const char *vs_entry, VkShaderModule_T *fs,
@@ -423,6 +424,8 @@ private:
   size_t c_stemp = 0;
   void *d_einfo = nullptr;  // per-triangle pair-emission descriptors
   size_t c_einfo = 0;
+  void *d_stamps = nullptr;  // device wall-clock stamps at stage boundaries
+  size_t c_stamps = 0;
   uint32_t pair_cap = 0;    // (tile, primitive) pair capacity of the bin buffers
   // last uploaded per-pass tables (skipped when a submission repeats them)
   std::vector<VaFrame> up_frames;
@@ -434,12 +437,12 @@ private:
   {
     const void *mod;
     uint32_t dims[8];
-    void *bufs[16];
+    void *bufs[20];
   };
   struct GraphSet
   {
     GraphKey key;
-    hipGraphExec_t exec[3] = {nullptr, nullptr, nullptr};  // setup + scan, binning, raster
+    hipGraphExec_t exec[2] = {nullptr, nullptr};  // setup + scan + binning, raster
   };
   std::vector<GraphSet> graph_cache;  // most recent first, at most 8
   size_t c_frames = 0, c_draws = 0, c_rec = 0, c_count = 0, c_offs = 0, c_keys = 0, c_vals = 0, c_keys2 = 0,

@@ -27,6 +27,7 @@ hipError_t va_fill_u64_async(void *p, uint64_t count, uint32_t lo, uint32_t hi, 
 #ifdef __cplusplus
 extern "C" {
 #endif
+hipError_t va_stamp_async(uint64_t *out, hipStream_t stream);
 hipError_t va_reduce_cols_async(const uint64_t *part, uint32_t rows, uint32_t cols, uint64_t *out, hipStream_t stream);
 #ifdef __cplusplus
 }

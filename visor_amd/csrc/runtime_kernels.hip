@@ -318,6 +318,22 @@ __global__ __launch_bounds__(256) void va_fill64_kernel(uint2 *p, uint64_t n, ui
     p[i] = v;
 }
 
+// Device wall clock (constant rate, hipDeviceAttributeWallClockRate kHz) at this point of the
+// stream: stage boundaries inside a captured graph, where event records give no timings.
+__global__ __launch_bounds__(64) void va_stamp_kernel(uint64_t *out)
+{
+  const uint64_t t = (uint64_t)wall_clock64();
+  if(threadIdx.x == 0)
+    out[0] = t;
+}
+
+extern "C" hipError_t va_stamp_async(uint64_t *out, hipStream_t stream)
+{
+  (void)hipGetLastError();  // report this launch's error only
+  hipLaunchKernelGGL(va_stamp_kernel, dim3(1), dim3(64), 0, stream, out);
+  return hipGetLastError();
+}
+
 static unsigned fill_blocks(uint64_t n)
 {
   uint64_t b = (n + 255) / 256;
