@@ -1,9 +1,9 @@
 #!/usr/bin/env python
 """Benchmark: batched vs plain per-draw execution of the 256-draw workload on one MI355X.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--tile 8|16|32]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--tile 0|8|16|32] [--no-serve] ...
 
-Workload (config 3): one 1920x1080 BGRA8 + D32 target, 256 draws of log-uniform 16..65536
+Workload: one 1920x1080 BGRA8 + D32 target, 256 draws of log-uniform 16..65536
 triangles (about 1.85M), per-draw MVP, 4 mipmapped 512x512 textures sampled trilinear repeat;
 even draws opaque (LESS), odd draws alpha-blended with depth writes off. Inputs (vertex, index,
 texture data) are resident in HBM; the target stays in HBM.
@@ -126,14 +126,14 @@ def main():
     except Exception:  # torch is plumbing only; the ICD synchronises its own stream
         sync = lambda: None
 
-    scene = scenes.config3()
+    scene = scenes.many_draws()
     ctx = render.Context()
     r = render.SceneRenderer(ctx, scene, host_visible_targets=False)
     ctx.set_mode(batched=True, tile=args.tile, graphs=not args.no_graphs)
     cb = r.record()
     r.cb = cb
 
-    print("[bench] config3: %d draws, %d triangles; warmup %d" % (len(scene.draws), scene.triangles, args.warmup),
+    print("[bench] many_draws: %d draws, %d triangles; warmup %d" % (len(scene.draws), scene.triangles, args.warmup),
           file=sys.stderr, flush=True)
     for _ in range(args.warmup):
         r.render()
@@ -184,7 +184,7 @@ def main():
         res = refrast.render(scene, threads=7, draws=scene.draws[:k], time_only=True)
         cpu = {"value": round(res["counters"]["pixels_written"] / res["seconds"] / 1e9, 6), "unit": "Gfragments/s",
                "cores": 8, "kind": "port",
-               "sample": "first %d of 256 draws of config3 (%d covered fragments, %.1f s), oracle threaded mode: "
+               "sample": "first %d of the 256 draws (%d covered fragments, %.1f s), oracle threaded mode: "
                          "7 workers + work-stealing submitter over 32x32 blocks" % (
                              k, res["counters"]["pixels_written"], res["seconds"])}
         print("[bench] cpu baseline %.4f Gfrag/s" % cpu["value"], file=sys.stderr, flush=True)
@@ -207,7 +207,7 @@ def main():
         "vs_baseline": None,
         "dtype": "fp32",
         "data": "synthetic",
-        "config": {"workload": "config3: 1920x1080 BGRA8+D32, 256 draws, %d triangles, trilinear repeat, "
+        "config": {"workload": "256 draws into 1920x1080 BGRA8+D32, %d triangles, trilinear repeat, "
                                "half alpha-blended" % scene.triangles,
                    "mode": "batched", "tile": tile, "chunk": stats_one["chunk"],
                    "launch_shape": "autotuned" if args.tile == 0 else "fixed", "fragments_per_frame": frags_per_frame,

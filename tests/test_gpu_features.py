@@ -40,7 +40,7 @@ def _check(ctx, s, ref=None, passes=None, batched=True):
 
 
 def _soup(ntris=300, w=96, h=80, seed=5, **pipe):
-    s = scenes.config1(ntris, w, h, seed=seed)
+    s = scenes.random_triangles(ntris, w, h, seed=seed)
     s.pipelines = {"main": dataclasses.replace(s.pipelines["main"], **pipe)}
     return s
 
@@ -155,6 +155,6 @@ SAMPLERS = [
 
 @pytest.mark.parametrize("k", range(len(SAMPLERS)))
 def test_sampler_modes(ctx, k):
-    s = scenes.config3(ndraws=10, width=160, height=120, tex_size=64, max_tris=1024, seed=21)
+    s = scenes.many_draws(ndraws=10, width=160, height=120, tex_size=64, max_tris=1024, seed=21)
     s.textures = [dataclasses.replace(t, **SAMPLERS[k]) for t in s.textures]
     _check(ctx, s)

@@ -32,8 +32,8 @@ def _render_gpu(ctx, scene, batched=True, tile=16):
 
 
 @pytest.mark.parametrize("batched", [True, False])
-def test_config1_bit_exact(ctx, batched):
-    s = scenes.config1()
+def test_random_triangles_bit_exact(ctx, batched):
+    s = scenes.random_triangles()
     ref = refrast.render(s)
     out = _render_gpu(ctx, s, batched=batched)
     _compare(s, out, ref)
@@ -45,9 +45,9 @@ def test_config1_bit_exact(ctx, batched):
     assert st["pixels_tested"] == c["pixels_tested"]
 
 
-def test_config2_small_bit_exact(ctx):
+def test_lit_sphere_small_bit_exact(ctx):
     """Indexed sphere, MVP + normal matrix, Lambert x mipmapped texture, clamp, back-face cull."""
-    s = scenes.config2(stacks=40, slices=64, width=320, height=180, tex_size=64)
+    s = scenes.lit_sphere(stacks=40, slices=64, width=320, height=180, tex_size=64)
     ref = refrast.render(s)
     out = _render_gpu(ctx, s)
     _compare(s, out, ref)
@@ -56,26 +56,26 @@ def test_config2_small_bit_exact(ctx):
 
 
 @pytest.mark.parametrize("tile", [8, 16, 32])
-def test_config3_small_bit_exact(ctx, tile):
+def test_many_draws_small_bit_exact(ctx, tile):
     """Many draws, per-draw MVP, trilinear repeat, alternating opaque / alpha-blended."""
-    s = scenes.config3(ndraws=24, width=320, height=240, tex_size=64, max_tris=2048)
+    s = scenes.many_draws(ndraws=24, width=320, height=240, tex_size=64, max_tris=2048)
     ref = refrast.render(s)
     out = _render_gpu(ctx, s, tile=tile)
     _compare(s, out, ref)
     assert out["stats"]["depth_passed"] == ref["counters"]["depth_passed"]
 
 
-def test_config5_small_bit_exact(ctx):
+def test_micro_triangles_small_bit_exact(ctx):
     """RGBA16F target, micro-triangles, 8 trilinear taps of an array texture, additive draws."""
-    s = scenes.config5(ntris=20000, ndraws=4, width=256, height=256, tex_size=64, layers=4)
+    s = scenes.micro_triangles(ntris=20000, ndraws=4, width=256, height=256, tex_size=64, layers=4)
     ref = refrast.render(s)
     out = _render_gpu(ctx, s)
     _compare(s, out, ref)
 
 
-def test_config3_full_batched_equals_plain(ctx):
+def test_many_draws_full_batched_equals_plain(ctx):
     """At full size: batched and plain per-draw execution give identical bytes."""
-    s = scenes.config3()
+    s = scenes.many_draws()
     a = _render_gpu(ctx, s, batched=True)
     b = _render_gpu(ctx, s, batched=False)
     assert np.array_equal(a["color"], b["color"])
@@ -86,9 +86,9 @@ def test_config3_full_batched_equals_plain(ctx):
 def test_graph_replays_match_eager(ctx):
     """A batched pass captured into a HIP graph and replayed, with other (eager, per-draw) work
     and host synchronisation between replays, gives the eager result on every replay."""
-    s = scenes.config3(ndraws=24, width=320, height=240, tex_size=64, max_tris=2048)
+    s = scenes.many_draws(ndraws=24, width=320, height=240, tex_size=64, max_tris=2048)
     r = render.SceneRenderer(ctx, s)
-    o = render.SceneRenderer(ctx, scenes.config1(300, 96, 96, seed=7))
+    o = render.SceneRenderer(ctx, scenes.random_triangles(300, 96, 96, seed=7))
     ctx.set_mode(batched=True, tile=8, graphs=False)
     r.render()
     ref_c, ref_d = r.read_color().copy(), r.read_depth().copy()
@@ -108,7 +108,7 @@ def test_autotuner_shapes_render_identical_bytes(ctx):
     """With the autotuner on, the first repetitions of a pass run under each candidate launch
     shape (tile size, waves per workgroup, LDS chunk); every repetition must give the same bytes,
     and the tuner settles on one candidate."""
-    s = scenes.config3(ndraws=24, width=320, height=240, tex_size=64, max_tris=2048)
+    s = scenes.many_draws(ndraws=24, width=320, height=240, tex_size=64, max_tris=2048)
     ref = refrast.render(s)
     r = render.SceneRenderer(ctx, s)
     ctx.set_mode(batched=True, tile=0, graphs=True)

@@ -68,12 +68,12 @@ def _digest(scene):
 
 
 def test_workloads_are_deterministic():
-    assert _digest(scenes.config1(200, 64, 64, seed=4)) == _digest(scenes.config1(200, 64, 64, seed=4))
-    assert _digest(scenes.config1(200, 64, 64, seed=4)) != _digest(scenes.config1(200, 64, 64, seed=5))
+    assert _digest(scenes.random_triangles(200, 64, 64, seed=4)) == _digest(scenes.random_triangles(200, 64, 64, seed=4))
+    assert _digest(scenes.random_triangles(200, 64, 64, seed=4)) != _digest(scenes.random_triangles(200, 64, 64, seed=5))
 
 
 def test_256_draw_workload_shape():
-    s = scenes.config3()
+    s = scenes.many_draws()
     assert (s.width, s.height, len(s.draws)) == (1920, 1080, 256)
     counts = [d.count // 3 for d in s.draws]
     assert min(counts) >= 16 and max(counts) <= 65536

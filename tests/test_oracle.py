@@ -326,15 +326,15 @@ def test_subresource_offsets():
 
 
 def test_threaded_mode_coverage_counters_match_serial():
-    s = scenes.config1(ntris=200, width=128, height=128, seed=9)
+    s = scenes.random_triangles(ntris=200, width=128, height=128, seed=9)
     a = refrast.render(s)["counters"]
     b = refrast.render(s, threads=4, time_only=True)["counters"]
     for k in ("tris_in", "tris_out", "blocks", "pixels_tested", "pixels_written"):
         assert a[k] == b[k], k
 
 
-def test_config1_expected_scale():
+def test_random_triangles_expected_scale():
     """1000 random clip-space triangles on 256x256: about 5.0M covered, 0.3M shaded."""
-    c = refrast.render(scenes.config1())["counters"]
+    c = refrast.render(scenes.random_triangles())["counters"]
     assert 4.5e6 < c["pixels_written"] < 5.5e6
     assert 2.0e5 < c["depth_passed"] < 4.0e5

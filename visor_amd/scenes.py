@@ -226,7 +226,7 @@ def interleave_pnu(pos, nrm, uv):
 
 # ---- configurations ------------------------------------------------------------------------------
 
-def config1(ntris=1000, width=256, height=256, seed=1):
+def random_triangles(ntris=1000, width=256, height=256, seed=1):
     """256x256 RGBA8 + D32, 1000 non-indexed random clip-space triangles, colour passthrough."""
     rng = Rng(seed)
     nv = 3 * ntris
@@ -238,7 +238,7 @@ def config1(ntris=1000, width=256, height=256, seed=1):
     col = rng.bytes(nv * 4).reshape(nv, 4)
     pipes = {"main": Pipeline("vs_mvp_pos4_col", "fs_color")}
     draws = [Draw("main", nv, 0, False, push_block(IDENTITY))]
-    return Scene("config1", width, height, interleave_pos_col(pos, col), 20, VATTRS_POS_COL, pipes, draws)
+    return Scene("random_triangles", width, height, interleave_pos_col(pos, col), 20, VATTRS_POS_COL, pipes, draws)
 
 
 def uv_sphere(stacks, slices, radius, rng=None, jitter=0.0, center=(0, 0, 0)):
@@ -279,7 +279,7 @@ def _orient_front_ccw(tri, pos, mvp, width, height):
     return tri
 
 
-def config2(stacks=250, slices=400, width=1920, height=1080, tex_size=1024, seed=2):
+def lit_sphere(stacks=250, slices=400, width=1920, height=1080, tex_size=1024, seed=2):
     """One indexed draw: 200k-triangle perturbed UV sphere, MVP + normal matrix, Lambert x
     mipmapped noise texture (bilinear, clamp), LESS, back-face cull."""
     rng = Rng(seed)
@@ -294,11 +294,11 @@ def config2(stacks=250, slices=400, width=1920, height=1080, tex_size=1024, seed
                   mipmap_mode=MIPMAP_NEAREST, max_lod=float(levels - 1))
     pipes = {"lit": Pipeline("vs_mvp_normal_uv", "fs_lambert_tex", cull=CULL_BACK)}
     draws = [Draw("lit", tri.size, 0, True, push_block(mvp, nmat), texture=0)]
-    return Scene("config2", width, height, interleave_pnu(pos, nrm, uv), 32, VATTRS_PNU, pipes, draws,
+    return Scene("lit_sphere", width, height, interleave_pnu(pos, nrm, uv), 32, VATTRS_PNU, pipes, draws,
                  index_data=tri.astype(np.uint32).tobytes(), textures=[tex])
 
 
-def config3(ndraws=256, width=1920, height=1080, tex_size=512, seed=3, min_tris=16, max_tris=65536):
+def many_draws(ndraws=256, width=1920, height=1080, tex_size=512, seed=3, min_tris=16, max_tris=65536):
     """256 draws of log-uniform 16..65536 triangles (about 2M), per-draw MVP, 4 trilinear repeat
     textures; even draws opaque LESS, odd draws alpha-blended (depth write off)."""
     rng = Rng(seed)
@@ -341,11 +341,11 @@ def config3(ndraws=256, width=1920, height=1080, tex_size=512, seed=3, min_tris=
         vbase += len(pos)
         ibase += tri.size
     vdata = interleave_pnu(np.concatenate(pos_all), np.concatenate(nrm_all), np.concatenate(uv_all))
-    return Scene("config3", width, height, vdata, 32, VATTRS_PNU, pipes, draws,
+    return Scene("many_draws", width, height, vdata, 32, VATTRS_PNU, pipes, draws,
                  index_data=np.concatenate(idx_all).tobytes(), textures=textures)
 
 
-def config5(ntris=4_000_000, ndraws=16, width=3840, height=2160, tex_size=2048, layers=4, seed=5):
+def micro_triangles(ntris=4_000_000, ndraws=16, width=3840, height=2160, tex_size=2048, layers=4, seed=5):
     """Micro-triangles: 4M indexed triangles of about 4 px in 50 depth layers on a 4K RGBA16F
     target; 8 trilinear taps of a 4-layer mipmapped array texture; 25% of draws additive."""
     rng = Rng(seed)
@@ -376,8 +376,9 @@ def config5(ntris=4_000_000, ndraws=16, width=3840, height=2160, tex_size=2048, 
     for d in range(ndraws):
         draws.append(Draw("add" if d % 4 == 3 else "opaque", per * 3, d * per * 3, True,
                           push_block(IDENTITY, np.eye(3)), texture=0))
-    return Scene("config5", width, height, interleave_pnu(pos.reshape(-1, 3), nrm, uv), 32, VATTRS_PNU, pipes, draws,
+    return Scene("micro_triangles", width, height, interleave_pnu(pos.reshape(-1, 3), nrm, uv), 32, VATTRS_PNU, pipes, draws,
                  index_data=idx.tobytes(), textures=[tex], color_format=FMT["R16G16B16A16_SFLOAT"])
 
 
-CONFIGS = {"config1": config1, "config2": config2, "config3": config3, "config5": config5}
+WORKLOADS = {"random_triangles": random_triangles, "lit_sphere": lit_sphere, "many_draws": many_draws,
+             "micro_triangles": micro_triangles}
