@@ -512,12 +512,15 @@ VA_INLINE void va_raster_tile(const VaDraw *__restrict__ draws, const VaFrame *_
 {
   constexpr u32 SB = T / 8, NW = SB * SB, NT = NW * 64u;
   constexpr u32 Q = VA_REC_STRIDE / 4u;
-  __shared__ uint4 s_rec4[VA_CHUNK * Q];
+  // Records in LDS at an odd word stride: lanes reading one field of different records then hit
+  // different banks (a stride of VA_REC_STRIDE, a multiple of 32 words, would put them all on one).
+  constexpr u32 LS = VA_REC_STRIDE + 1u;
+  __shared__ u32 s_recw[VA_CHUNK * LS];
   __shared__ u32 s_tri[VA_CHUNK];
   __shared__ u16 s_flags[VA_CHUNK];   // per-record fixed-function state (va_rec_flags)
   __shared__ u16 s_job[NW][64];       // jobs of one shading round: record << 6 | pixel
   __shared__ float4 s_res[NW][64];    // fragment program results of one shading round
-  const u32 *s_rec = (const u32 *)s_rec4;
+  const u32 *s_rec = s_recw;
 
   const VaFrame &fr = frames[va_find_frame(frames, nframes, tile)];
   const u32 lt = tile - fr.tile_base;
@@ -581,18 +584,23 @@ VA_INLINE void va_raster_tile(const VaDraw *__restrict__ draws, const VaFrame *_
     for(u32 w = threadIdx.x; w < n * Q; w += NT)
     {
       const u32 k = w / Q, q = w - k * Q;
-      s_rec4[w] = ((const uint4 *)(rec + (u64)s_tri[k] * VA_REC_STRIDE))[q];
+      const uint4 v = ((const uint4 *)(rec + (u64)s_tri[k] * VA_REC_STRIDE))[q];
+      u32 *dst = s_recw + k * LS + 4u * q;
+      dst[0] = v.x;
+      dst[1] = v.y;
+      dst[2] = v.z;
+      dst[3] = v.w;
     }
     __syncthreads();
     for(u32 k = threadIdx.x; k < n; k += NT)
-      s_flags[k] = (u16)(s_rec[k * VA_REC_STRIDE + VA_R_DRAW] >> VA_DRAW_BITS);
+      s_flags[k] = (u16)(s_rec[k * LS + VA_R_DRAW] >> VA_DRAW_BITS);
     __syncthreads();
 
 #if VA_FS_KILL
     // fragment programs that may discard decide depth writes: shade in place, in order
     for(u32 k = 0; k < n; ++k)
     {
-      const u32 *r = s_rec + k * VA_REC_STRIDE;
+      const u32 *r = s_rec + k * LS;
       const u32 bmin = r[VA_R_BBMIN], bmax = r[VA_R_BBMAX];
       const i32 minx = (i32)(bmin & 0xffffu), miny = (i32)(bmin >> 16);
       const i32 maxx = (i32)(bmax & 0xffffu), maxy = (i32)(bmax >> 16);
@@ -628,7 +636,7 @@ VA_INLINE void va_raster_tile(const VaDraw *__restrict__ draws, const VaFrame *_
     bool big = false;
     if(lane < n)
     {
-      const u32 *r = s_rec + lane * VA_REC_STRIDE;
+      const u32 *r = s_rec + lane * LS;
       const u32 bmin = r[VA_R_BBMIN], bmax = r[VA_R_BBMAX];
       const i32 x0 = max((i32)(bmin & 0xffffu), bx), y0 = max((i32)(bmin >> 16), by);
       const i32 x1 = min(min((i32)(bmax & 0xffffu), bx + 8), (i32)W);
@@ -656,7 +664,7 @@ VA_INLINE void va_raster_tile(const VaDraw *__restrict__ draws, const VaFrame *_
     for(u64 bigs = __ballot(big); bigs; bigs &= bigs - 1ull)
     {
       const u32 k = (u32)__builtin_ctzll(bigs);
-      const u32 *r = s_rec + k * VA_REC_STRIDE;
+      const u32 *r = s_rec + k * LS;
       const u32 bmin = r[VA_R_BBMIN], bmax = r[VA_R_BBMAX];
       const bool cov = inside && va_covered(r, px, py) && px >= (i32)(bmin & 0xffffu) && px < (i32)(bmax & 0xffffu) &&
                        py >= (i32)(bmin >> 16) && py < (i32)(bmax >> 16);
@@ -680,7 +688,7 @@ VA_INLINE void va_raster_tile(const VaDraw *__restrict__ draws, const VaFrame *_
       float z = 0.0f;
       if(has_depth && (op != 7u || (f & VA_F_DWRITE)))
       {
-        va_cov_depth(s_rec + k * VA_REC_STRIDE, px, py, &z);
+        va_cov_depth(s_rec + k * LS, px, py, &z);
         if(op != 7u)
           passed = va_depth_pass(op, z, dep);
       }
@@ -722,7 +730,7 @@ VA_INLINE void va_raster_tile(const VaDraw *__restrict__ draws, const VaFrame *_
       {
         const u32 job = s_job[wave][lane];
         const u32 k = job >> 6, p = job & 63u;
-        const u32 *r = s_rec + k * VA_REC_STRIDE;
+        const u32 *r = s_rec + k * LS;
         const i32 x = bx + (i32)(p & 7u), y = by + (i32)(p >> 3);
         float b[3][4];
         va_bary(r, x, y, b[0], nullptr);
