@@ -516,7 +516,6 @@ VA_INLINE void va_raster_tile(const VaDraw *__restrict__ draws, const VaFrame *_
   // different banks (a stride of VA_REC_STRIDE, a multiple of 32 words, would put them all on one).
   constexpr u32 LS = VA_REC_STRIDE + 1u;
   __shared__ u32 s_recw[VA_CHUNK * LS];
-  __shared__ u32 s_tri[VA_CHUNK];
   __shared__ u16 s_flags[VA_CHUNK];   // per-record fixed-function state (va_rec_flags)
   __shared__ u16 s_job[NW][64];       // jobs of one shading round: record << 6 | pixel
   __shared__ float4 s_res[NW][64];    // fragment program results of one shading round
@@ -574,27 +573,60 @@ VA_INLINE void va_raster_tile(const VaDraw *__restrict__ draws, const VaFrame *_
   }
 
   u32 n_cov = 0, n_pass = 0, n_blend = 0, n_dw = 0;
-  for(u32 cb = beg; cb < en; cb += VA_CHUNK)
-  {
-    const u32 n = min(VA_CHUNK, en - cb);
-    __syncthreads();  // the previous chunk is consumed by every wave
-    for(u32 i = threadIdx.x; i < n; i += NT)
-      s_tri[i] = bins[cb + i];
-    __syncthreads();
-    for(u32 w = threadIdx.x; w < n * Q; w += NT)
+  // Staging is software-pipelined through registers: while chunk c is processed, the records of
+  // chunk c + 1 (whose bin entries arrived an iteration earlier) and the bin entries of chunk
+  // c + 2 are in flight. Thread slot j of a chunk covers 16-byte group w = thread + j * NT.
+  constexpr u32 JMAX = (VA_CHUNK * Q + NT - 1u) / NT;
+  const u32 nchunks = (en - beg + VA_CHUNK - 1u) / VA_CHUNK;
+  auto chunk_len = [&](u32 c) { return c < nchunks ? min(VA_CHUNK, en - (beg + c * VA_CHUNK)) : 0u; };
+  u32 tri_next[JMAX];
+  uint4 rec_next[JMAX];
+  auto load_tris = [&](u32 c) {
+    const u32 nn = chunk_len(c), c0 = beg + c * VA_CHUNK;
+#pragma unroll
+    for(u32 j = 0; j < JMAX; j++)
     {
-      const u32 k = w / Q, q = w - k * Q;
-      const uint4 v = ((const uint4 *)(rec + (u64)s_tri[k] * VA_REC_STRIDE))[q];
-      u32 *dst = s_recw + k * LS + 4u * q;
-      dst[0] = v.x;
-      dst[1] = v.y;
-      dst[2] = v.z;
-      dst[3] = v.w;
+      const u32 w = threadIdx.x + j * NT;
+      tri_next[j] = w < nn * Q ? bins[c0 + w / Q] : 0u;
+    }
+  };
+  auto load_recs = [&](u32 c) {
+    const u32 nn = chunk_len(c);
+#pragma unroll
+    for(u32 j = 0; j < JMAX; j++)
+    {
+      const u32 w = threadIdx.x + j * NT;
+      if(w < nn * Q)
+        rec_next[j] = ((const uint4 *)(rec + (u64)tri_next[j] * VA_REC_STRIDE))[w % Q];
+    }
+  };
+  load_tris(0);
+  load_recs(0);
+  load_tris(1);
+  for(u32 c = 0; c < nchunks; c++)
+  {
+    const u32 n = chunk_len(c);
+    __syncthreads();  // the previous chunk is consumed by every wave
+#pragma unroll
+    for(u32 j = 0; j < JMAX; j++)
+    {
+      const u32 w = threadIdx.x + j * NT;
+      if(w < n * Q)
+      {
+        const u32 k = w / Q, q = w - k * Q;
+        u32 *dst = s_recw + k * LS + 4u * q;
+        dst[0] = rec_next[j].x;
+        dst[1] = rec_next[j].y;
+        dst[2] = rec_next[j].z;
+        dst[3] = rec_next[j].w;
+      }
     }
     __syncthreads();
     for(u32 k = threadIdx.x; k < n; k += NT)
       s_flags[k] = (u16)(s_rec[k * LS + VA_R_DRAW] >> VA_DRAW_BITS);
     __syncthreads();
+    load_recs(c + 1);  // addresses from the bin entries loaded last iteration
+    load_tris(c + 2);
 
 #if VA_FS_KILL
     // fragment programs that may discard decide depth writes: shade in place, in order
