@@ -83,8 +83,8 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=12, help="untimed frames; they also let the autotuner try "
-                    "its 5 launch shapes twice")
+    ap.add_argument("--warmup", type=int, default=16, help="untimed frames; they also let the autotuner try "
+                    "its 7 launch shapes twice")
     ap.add_argument("--plain-steps", type=int, default=3)
     ap.add_argument("--tile", type=int, default=0, help="8, 16 or 32 fixes the tile size; 0 = autotuned")
     ap.add_argument("--chunk", type=int, default=0, help="16..64 fixes the LDS chunk (with --tile); 0 = default")
@@ -209,7 +209,7 @@ def main():
         "data": "synthetic",
         "config": {"workload": "256 draws into 1920x1080 BGRA8+D32, %d triangles, trilinear repeat, "
                                "half alpha-blended" % scene.triangles,
-                   "mode": "batched", "tile": tile, "chunk": stats_one["chunk"],
+                   "mode": "batched", "tile": tile, "chunk": stats_one["chunk"], "persistent": bool(stats_one["persistent"]),
                    "launch_shape": "autotuned" if args.tile == 0 else "fixed", "fragments_per_frame": frags_per_frame,
                    "shaded_per_frame": stats_one["depth_passed"]},
         "plain_per_draw_gfrag_s": round(plain, 4) if plain else None,

@@ -57,7 +57,8 @@ typedef struct VhStats {
   uint64_t shaded_dwrite;   /* shaded fragments that wrote depth                           */
   uint64_t graph_launches;  /* batched passes replayed from a captured HIP graph             */
   uint64_t tile_size;       /* launch shape of the last pass: tile side (8, 16, 32) ...      */
-  uint64_t chunk;           /* ... and bin entries staged in LDS per round                  */
+  uint64_t chunk;           /* ... bin entries staged in LDS per round ...                  */
+  uint64_t persistent;      /* ... and 1 when the raster grid was the persistent form       */
 } VhStats;
 
 /* 1 when a HIP device is usable. */

@@ -106,18 +106,18 @@ def test_graph_replays_match_eager(ctx):
 
 def test_autotuner_shapes_render_identical_bytes(ctx):
     """With the autotuner on, the first repetitions of a pass run under each candidate launch
-    shape (tile size, waves per workgroup, LDS chunk); every repetition must give the same bytes,
-    and the tuner settles on one candidate."""
+    shape (tile size, waves per workgroup, LDS chunk, persistent or one workgroup per tile); every
+    repetition must give the same bytes, and the tuner settles on one candidate."""
     s = scenes.many_draws(ndraws=24, width=320, height=240, tex_size=64, max_tris=2048)
     ref = refrast.render(s)
     r = render.SceneRenderer(ctx, s)
     ctx.set_mode(batched=True, tile=0, graphs=True)
     shapes = []
-    for i in range(12):
+    for i in range(16):
         r.render()
         _compare(s, {"color": r.read_color(), "depth": r.read_depth()}, ref)
         st = ctx.stats(0)
-        shapes.append((st["tile_size"], st["chunk"]))
-    assert len(set(shapes[:10])) == 5
-    assert shapes[10] == shapes[11] and shapes[10] in set(shapes[:10])
+        shapes.append((st["tile_size"], st["chunk"], st["persistent"]))
+    assert len(set(shapes[:14])) == 7
+    assert shapes[14] == shapes[15] and shapes[14] in set(shapes[:14])
     ctx.set_mode(batched=True, tile=8)

@@ -363,9 +363,11 @@ std::shared_ptr<PipelineModule> Device::GetModule(const std::vector<std::shared_
     err = std::string("hipModuleLoadData: ") + hipGetErrorString(e);
     return nullptr;
   }
-  const char *names[] = {"va_setup", "va_emit", "va_ranges", "va_raster_8", "va_raster_16", "va_raster_32"};
-  hipFunction_t *fns[] = {&pm->setup, &pm->emit, &pm->ranges, &pm->raster[0], &pm->raster[1], &pm->raster[2]};
-  for(int i = 0; i < 6; i++)
+  const char *names[] = {"va_setup",    "va_emit",      "va_ranges",    "va_raster_8", "va_raster_16",
+                         "va_raster_32", "va_raster_p8", "va_raster_p16", "va_raster_p32"};
+  hipFunction_t *fns[] = {&pm->setup,     &pm->emit,        &pm->ranges,      &pm->raster[0], &pm->raster[1],
+                          &pm->raster[2], &pm->raster_p[0], &pm->raster_p[1], &pm->raster_p[2]};
+  for(int i = 0; i < 9; i++)
   {
     e = hipModuleGetFunction(fns[i], pm->module, names[i]);
     if(e != hipSuccess)
@@ -373,6 +375,19 @@ std::shared_ptr<PipelineModule> Device::GetModule(const std::vector<std::shared_
       err = std::string("hipModuleGetFunction(") + names[i] + "): " + hipGetErrorString(e);
       return nullptr;
     }
+  }
+  // persistent raster grids: resident workgroups per CU x CUs
+  for(int i = 0; i < 3; i++)
+  {
+    int per_cu = 0;
+    const int threads = (8 << i) / 8 * ((8 << i) / 8) * 64;
+    if(hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, pm->raster_p[i], threads, 0) != hipSuccess ||
+       per_cu < 1)
+    {
+      (void)hipGetLastError();
+      per_cu = 1;
+    }
+    pm->raster_p_grid[i] = (uint32_t)per_cu * (uint32_t)std::max(props.multiProcessorCount, 1);
   }
   for(auto &p : ps)
     pm->slot_keys.push_back(p->key);

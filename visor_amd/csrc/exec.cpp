@@ -53,6 +53,8 @@ Queue::Queue()
     tiles.tile_shift = sh;
     autotune = false;
   }
+  if(const char *pp = getenv("VISOR_PERSISTENT"))
+    tiles.persistent = !strcmp(pp, "1") ? 1u : 0u;
   if(const char *at = getenv("VISOR_AUTOTUNE"))
     if(!strcmp(at, "0"))
       autotune = false;
@@ -590,7 +592,8 @@ static int BitsFor(uint32_t n)
 // timed by the stage events; after two rounds over the candidates the fastest is kept.
 
 static const TileConfig kTuneCandidates[] = {
-    {3, 64, 16}, {3, 32, 16}, {4, 64, 16}, {4, 32, 16}, {5, 64, 16},
+    {3, 64, 16, 0}, {3, 32, 16, 0}, {4, 64, 16, 0}, {4, 32, 16, 0}, {5, 64, 16, 0},
+    {3, 32, 16, 1}, {4, 32, 16, 1},
 };
 static const int kTuneRounds = 2;
 
@@ -863,8 +866,19 @@ VkResult Queue::RunPass(const std::vector<Op *> &passes, bool fused_clear, ExecS
         uint32_t *st_ = (uint32_t *)d_start, *en = (uint32_t *)d_start + 1;
         uint32_t use_fused = fused_clear ? 1u : 0u;
         void *args[] = {&dd, &df, (void *)&nframes, &rec, &bins, &st_, &en, &tile_total, &use_fused, &part, &total_dev, &cap};
-        hipFunction_t k = mod->raster[ts - 3];
         uint32_t threads = (T / 8) * (T / 8) * 64;
+        if(cfg.persistent)
+        {
+          uint32_t *tile_counter = (uint32_t *)d_stamps + 8;
+          VA_TRY(va_fill_u32_async(tile_counter, 1, 0u, stream));
+          void *pargs[] = {&dd, &df,   (void *)&nframes, &rec,        &bins, &st_,         &en,
+                           &tile_total, &use_fused, &part, &total_dev, &cap, &tile_counter};
+          const uint32_t grid = std::min(mod->raster_p_grid[ts - 3], std::max(tile_total, 1u));
+          VA_TRY(hipModuleLaunchKernel(mod->raster_p[ts - 3], grid, 1, 1, threads, 1, 1, 0, stream, pargs, nullptr));
+          VA_TRY(va_reduce_cols_async(part, tile_total, 4, (uint64_t *)&ctr->pixels_written, stream));
+          return hipSuccess;
+        }
+        hipFunction_t k = mod->raster[ts - 3];
         VA_TRY(hipModuleLaunchKernel(k, tile_total, 1, 1, threads, 1, 1, 0, stream, args, nullptr));
         VA_TRY(va_reduce_cols_async(part, tile_total, 4, (uint64_t *)&ctr->pixels_written, stream));
       }
@@ -886,6 +900,7 @@ VkResult Queue::RunPass(const std::vector<Op *> &passes, bool fused_clear, ExecS
       key.dims[4] = nframes;
       key.dims[5] = cap;
       key.dims[6] = fused_clear ? 1u : 0u;
+      key.dims[7] = cfg.persistent;
       void *bufs[] = {d_frames, d_draws, d_rec, d_count, d_offs, d_keys, d_vals, d_keys2, d_vals2,
                       d_stemp, d_temp, d_start, d_einfo, d_ctr, d_blocks, d_part, d_stamps};
       static_assert(sizeof(bufs) <= sizeof(key.bufs), "graph key holds every buffer");

@@ -1328,6 +1328,7 @@ int vh_get_stats(VkQueue queue, int which, VhStats *out)
   out->graph_launches = s.graph_launches;
   out->tile_size = 1ull << s.last_cfg.tile_shift;
   out->chunk = s.last_cfg.chunk;
+  out->persistent = s.last_cfg.persistent;
   return 0;
 }
 

@@ -263,7 +263,9 @@ struct PipelineModule
   std::string key;
   hipModule_t module = nullptr;
   hipFunction_t setup = nullptr, emit = nullptr, ranges = nullptr;
-  hipFunction_t raster[3] = {nullptr, nullptr, nullptr};  // tile 8, 16, 32
+  hipFunction_t raster[3] = {nullptr, nullptr, nullptr};   // tile 8, 16, 32
+  hipFunction_t raster_p[3] = {nullptr, nullptr, nullptr};  // persistent forms
+  uint32_t raster_p_grid[3] = {0, 0, 0};                     // workgroups that fill the GPU
   uint32_t rec_stride = 0;  // words
   std::vector<std::string> slot_keys;
   double compile_seconds = 0.0;
@@ -322,9 +324,10 @@ struct TileConfig
   uint32_t tile_shift = 3;  // 3, 4, 5 -> 8, 16, 32 pixel tiles
   uint32_t chunk = 64;
   uint32_t small = 16;
+  uint32_t persistent = 0;  // 1: one GPU-filling grid taking tiles from a counter
   bool operator==(const TileConfig &o) const
   {
-    return tile_shift == o.tile_shift && chunk == o.chunk && small == o.small;
+    return tile_shift == o.tile_shift && chunk == o.chunk && small == o.small && persistent == o.persistent;
   }
 };
 
@@ -424,7 +427,7 @@ private:
   size_t c_stemp = 0;
   void *d_einfo = nullptr;  // per-triangle pair-emission descriptors
   size_t c_einfo = 0;
-  void *d_stamps = nullptr;  // device wall-clock stamps at stage boundaries
+  void *d_stamps = nullptr;  // device wall-clock stamps at stage boundaries (+ the tile counter)
   size_t c_stamps = 0;
   uint32_t pair_cap = 0;    // (tile, primitive) pair capacity of the bin buffers
   // last uploaded per-pass tables (skipped when a submission repeats them)

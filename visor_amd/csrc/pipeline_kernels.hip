@@ -824,3 +824,32 @@ VA_INLINE void va_raster_tile(const VaDraw *__restrict__ draws, const VaFrame *_
 VA_RASTER_KERNEL(8)
 VA_RASTER_KERNEL(16)
 VA_RASTER_KERNEL(32)
+
+// Persistent form: a grid that fills the GPU once; each workgroup takes tiles from a counter
+// (zeroed before the launch) until none are left, so every wave reaches the exit.
+#define VA_RASTER_PERSISTENT_KERNEL(T)                                                                        \
+  extern "C" __global__ __launch_bounds__((T / 8) * (T / 8) * 64) void va_raster_p##T(                       \
+      const VaDraw *__restrict__ draws, const VaFrame *__restrict__ frames, u32 nframes,                    \
+      const u32 *__restrict__ rec, const u32 *__restrict__ bins, const u32 *__restrict__ bin_start,          \
+      const u32 *__restrict__ bin_end, u32 ntiles, u32 use_fused_clear, u64 *__restrict__ part,              \
+      const u32 *__restrict__ pair_total, u32 pair_cap, u32 *__restrict__ tile_counter)                       \
+  {                                                                                                           \
+    __shared__ u32 s_next;                                                                                    \
+    if(*pair_total > pair_cap)                                                                                \
+      return;                                                                                                 \
+    for(;;)                                                                                                   \
+    {                                                                                                         \
+      if(threadIdx.x == 0u)                                                                                   \
+        s_next = atomicAdd(tile_counter, 1u);                                                                 \
+      __syncthreads();                                                                                        \
+      const u32 tile = s_next;                                                                                \
+      __syncthreads();                                                                                        \
+      if(tile >= ntiles)                                                                                      \
+        break;                                                                                                \
+      va_raster_tile<T>(draws, frames, nframes, rec, bins, bin_start, bin_end, use_fused_clear, part, tile);   \
+    }                                                                                                         \
+  }
+
+VA_RASTER_PERSISTENT_KERNEL(8)
+VA_RASTER_PERSISTENT_KERNEL(16)
+VA_RASTER_PERSISTENT_KERNEL(32)

@@ -229,7 +229,8 @@ class VhStats(C.Structure):
                 ("raster_launches", u64), ("triangles", u64), ("bin_pairs", u64), ("tiles", u64), ("draws", u64),
                 ("tris_in", u64), ("tris_out", u64), ("blocks", u64), ("pixels_tested", u64), ("pixels_written", u64),
                 ("depth_passed", u64), ("shaded_blend", u64), ("shaded_dwrite", u64),
-                ("graph_launches", u64), ("tile_size", u64), ("chunk", u64)]
+                ("graph_launches", u64), ("tile_size", u64), ("chunk", u64),
+                ("persistent", u64)]
 
 
 class VhServerConfig(C.Structure):
