@@ -182,8 +182,10 @@ def main():
         from oracle import refrast
         k = min(args.cpu_draws, len(scene.draws))
         res = refrast.render(scene, threads=7, draws=scene.draws[:k], time_only=True)
+        ser = refrast.render(scene, threads=0, draws=scene.draws[:k], time_only=True)
         cpu = {"value": round(res["counters"]["pixels_written"] / res["seconds"] / 1e9, 6), "unit": "Gfragments/s",
                "cores": 8, "kind": "port",
+               "serial_one_core_value": round(ser["counters"]["pixels_written"] / ser["seconds"] / 1e9, 6),
                "sample": "first %d of the 256 draws (%d covered fragments, %.1f s), oracle threaded mode: "
                          "7 workers + work-stealing submitter over 32x32 blocks" % (
                              k, res["counters"]["pixels_written"], res["seconds"])}

@@ -121,3 +121,36 @@ def test_autotuner_shapes_render_identical_bytes(ctx):
     assert len(set(shapes[:14])) == 7
     assert shapes[14] == shapes[15] and shapes[14] in set(shapes[:14])
     ctx.set_mode(batched=True, tile=8)
+
+
+def test_lit_sphere_full_bit_exact(ctx):
+    """Full size: 200k-triangle indexed sphere at 1920x1080, against the oracle."""
+    s = scenes.lit_sphere()
+    ref = refrast.render(s)
+    out = _render_gpu(ctx, s, tile=8)
+    _compare(s, out, ref)
+    assert out["stats"]["depth_passed"] == ref["counters"]["depth_passed"]
+
+
+def test_many_draws_full_bit_exact(ctx):
+    """The 256-draw workload at full size (1.85M triangles, 1920x1080) against the oracle, with
+    the autotuner choosing the launch shape."""
+    s = scenes.many_draws()
+    ref = refrast.render(s)
+    r = render.SceneRenderer(ctx, s)
+    ctx.set_mode(batched=True, tile=0, graphs=True)
+    r.render()
+    _compare(s, {"color": r.read_color(), "depth": r.read_depth()}, ref)
+    st = ctx.stats(0)
+    assert st["pixels_written"] == ref["counters"]["pixels_written"]
+    assert st["depth_passed"] == ref["counters"]["depth_passed"]
+    ctx.set_mode(batched=True, tile=8)
+
+
+def test_micro_triangles_4k_bit_exact(ctx):
+    """3840x2160 RGBA16F target, 1M micro-triangles (a quarter of the full count), 8 trilinear
+    taps of a 2048^2 x 4 array texture, a quarter of the draws additive, against the oracle."""
+    s = scenes.micro_triangles(ntris=1_000_000)
+    ref = refrast.render(s)
+    out = _render_gpu(ctx, s, tile=8)
+    _compare(s, out, ref)
