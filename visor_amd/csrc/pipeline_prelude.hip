@@ -54,6 +54,9 @@ VA_INLINE float va_ceil(float a) { return ceilf(a); }
 VA_INLINE df va_ceil(df a) { return df(ceilf(a.c), ceilf(a.x), ceilf(a.y)); }
 VA_INLINE float va_trunc(float a) { return truncf(a); }
 VA_INLINE df va_trunc(df a) { return df(truncf(a.c), truncf(a.x), truncf(a.y)); }
+// round to nearest, halfway cases to even (GLSL roundEven; also used for round)
+VA_INLINE float va_rint(float a) { return rintf(a); }
+VA_INLINE df va_rint(df a) { return df(rintf(a.c), rintf(a.x), rintf(a.y)); }
 VA_INLINE float va_fabs(float a) { return fabsf(a); }
 VA_INLINE df va_fabs(df a) { return df(fabsf(a.c), fabsf(a.x), fabsf(a.y)); }
 VA_INLINE float va_sin(float a) { return sinf(a); }

@@ -929,7 +929,7 @@ bool Translator::EmitExt(Frame &f, const Inst &in, Val &r)
           case G_Exp2: e = "va_exp2(" + x + ")"; break;
           case G_Log2: e = "va_log2f(" + x + ")"; break;
           case G_Fract: e = x + " - va_floor(" + x + ")"; break;
-          case G_Round: case G_RoundEven: e = "va_floor(" + x + " + F(0.5f))"; break;
+          case G_Round: case G_RoundEven: e = "va_rint(" + x + ")"; break;
           case G_FSign:
             e = "va_sel(cv(" + x + ") > 0.0f, F(1.0f), va_sel(cv(" + x + ") < 0.0f, F(-1.0f), F(0.0f)))";
             break;
