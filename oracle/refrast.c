@@ -126,6 +126,78 @@ float rr_log2(float x)
 /* ---------------------------------------------------------------------------------------- */
 /* Texture sampling                                                                          */
 
+/* BC2 / BC3 block decode, as the reference's vendored decoder does it (3rdparty/decompress.c,
+ * Dobell / Chajdas, MIT; called from texture_sampling.cpp:100-103). Output is 16 RGBA8 texels in
+ * row-major order. Two traits of that decoder are kept: BC2's colour block follows BC1's rule
+ * (three colours and black when c0 <= c1, decompress.c:141-206, 330-350), while BC3's colour
+ * block always uses four colours (decompress.c:300-318). */
+
+/* n-bit endpoint channel to 8 bits (decompress.c:122-135): t = v * 255 + 2^(n-1), (t / 2^n + t) / 2^n. */
+static uint32_t bc_widen(uint32_t v, uint32_t nbits)
+{
+  const uint32_t s = 1u << nbits;
+  const uint32_t t = v * 255u + (s >> 1);
+  return (t / s + t) / s;
+}
+
+static void bc_colour(const uint8_t *blk, int bc1_rule, const uint8_t alpha[16], uint8_t out[64])
+{
+  const uint32_t c0 = (uint32_t)blk[0] | (uint32_t)blk[1] << 8;
+  const uint32_t c1 = (uint32_t)blk[2] | (uint32_t)blk[3] << 8;
+  const uint32_t code = (uint32_t)blk[4] | (uint32_t)blk[5] << 8 | (uint32_t)blk[6] << 16 | (uint32_t)blk[7] << 24;
+  uint32_t pal[4][3];
+  pal[0][0] = bc_widen(c0 >> 11, 5); pal[0][1] = bc_widen((c0 >> 5) & 63u, 6); pal[0][2] = bc_widen(c0 & 31u, 5);
+  pal[1][0] = bc_widen(c1 >> 11, 5); pal[1][1] = bc_widen((c1 >> 5) & 63u, 6); pal[1][2] = bc_widen(c1 & 31u, 5);
+  const int four = !bc1_rule || c0 > c1;
+  for(int k = 0; k < 3; k++)
+  {
+    pal[2][k] = four ? (2u * pal[0][k] + pal[1][k]) / 3u : (pal[0][k] + pal[1][k]) / 2u;
+    pal[3][k] = four ? (pal[0][k] + 2u * pal[1][k]) / 3u : 0u;
+  }
+  for(int i = 0; i < 16; i++)
+  {
+    const uint32_t sel = (code >> (2 * i)) & 3u;
+    out[i * 4 + 0] = (uint8_t)pal[sel][0];
+    out[i * 4 + 1] = (uint8_t)pal[sel][1];
+    out[i * 4 + 2] = (uint8_t)pal[sel][2];
+    out[i * 4 + 3] = alpha[i];
+  }
+}
+
+void rr_decode_bc_block(uint32_t format, const uint8_t blk[16], uint8_t out[64])
+{
+  uint8_t alpha[16];
+  if(format == RR_FMT_BC2_UNORM_BLOCK)
+  {
+    /* 4-bit explicit alpha, row r in bytes 2r..2r+1, texel c in bits 4c.. (decompress.c:335-345) */
+    for(int i = 0; i < 16; i++)
+    {
+      const uint32_t row = (uint32_t)blk[2 * (i >> 2)] | (uint32_t)blk[2 * (i >> 2) + 1] << 8;
+      alpha[i] = (uint8_t)(((row >> (4 * (i & 3))) & 15u) * 17u);
+    }
+    bc_colour(blk + 8, 1, alpha, out);
+    return;
+  }
+  /* BC3: two 8-bit endpoints and sixteen 3-bit selectors from byte 2 (decompress.c:245-298) */
+  const uint32_t a0 = blk[0], a1 = blk[1];
+  uint64_t sel = 0;
+  for(int k = 0; k < 6; k++)
+    sel |= (uint64_t)blk[2 + k] << (8 * k);
+  for(int i = 0; i < 16; i++)
+  {
+    const uint32_t k = (uint32_t)(sel >> (3 * i)) & 7u;
+    uint32_t a;
+    if(k == 0) a = a0;
+    else if(k == 1) a = a1;
+    else if(a0 > a1) a = ((8u - k) * a0 + (k - 1u) * a1) / 7u;
+    else if(k == 6) a = 0;
+    else if(k == 7) a = 255;
+    else a = ((6u - k) * a0 + (k - 1u) * a1) / 5u;
+    alpha[i] = (uint8_t)a;
+  }
+  bc_colour(blk + 8, 0, alpha, out);
+}
+
 /* reference texture_sampling.cpp:5-137: an 8-entry per-thread LRU of 4x4 texel blocks. It is a
  * pure memo (a hit returns exactly what a direct fetch /255 returns); kept for the CPU timing. */
 typedef struct tc_entry {
@@ -195,19 +267,35 @@ static const float *tc_fetch(const rr_image *tex, uint64_t byte_offs, uint32_t r
   e->base = base;
   e->tlx = tlx;
   e->tly = tly;
-  /* The reference fills the whole 4x4 block, reading past the row or image end at the edges;
-   * here only in-bounds texels are read (the others are never returned for in-bounds x, y). */
+  /* Bytes past the end of the image read as 0 (the reference reads whatever follows). */
+  const uint64_t img_bytes = rr_subresource_offset(tex, 0, tex->array_layers);
+  const uint64_t boff = (uint64_t)(base - tex->pixels);
+  if(tex->format == RR_FMT_BC2_UNORM_BLOCK || tex->format == RR_FMT_BC3_UNORM_BLOCK)
+  {
+    /* reference texture_sampling.cpp:95-118: decode the 4x4 block holding the quad; the row
+     * pitch in blocks is the width >> 2 of the sampled subresource. */
+    uint8_t blk[16], rgba[64];
+    const uint64_t at = boff + ((uint64_t)(tly >> 2) * (row_w >> 2) + (uint32_t)(tlx >> 2)) * 16u;
+    for(int k = 0; k < 16; k++)
+      blk[k] = at + k < img_bytes ? tex->pixels[at + k] : 0;
+    rr_decode_bc_block(tex->format, blk, rgba);
+    for(int i = 0; i < 16; i++)
+      for(int c = 0; c < 4; c++)
+        e->px[i >> 2][i & 3][c] = (float)rgba[i * 4 + c] / 255.0f;
+    return e->px[y & 3][x & 3];
+  }
+  /* reference texture_sampling.cpp:121-133: 4 bytes at (y * width + x) * bpp, whatever the
+   * format. The reference fills the whole 4x4 block, reading past the row or image end at the
+   * edges; here only in-bounds texels are read (the others are never returned for in-bounds x, y). */
   for(int r = 0; r < 4; r++)
     for(int c = 0; c < 4; c++)
     {
       uint32_t xx = (uint32_t)(tlx + c), yy = (uint32_t)(tly + r);
       if(xx < row_w && yy < row_h)
       {
-        const uint8_t *p = base + ((uint64_t)yy * row_w + xx) * 4;
-        e->px[r][c][0] = (float)p[0] / 255.0f;
-        e->px[r][c][1] = (float)p[1] / 255.0f;
-        e->px[r][c][2] = (float)p[2] / 255.0f;
-        e->px[r][c][3] = (float)p[3] / 255.0f;
+        const uint64_t at = boff + ((uint64_t)yy * row_w + xx) * tex->bpp;
+        for(int k = 0; k < 4; k++)
+          e->px[r][c][k] = (float)(at + k < img_bytes ? tex->pixels[at + k] : 0) / 255.0f;
       }
       else
         e->px[r][c][0] = e->px[r][c][1] = e->px[r][c][2] = e->px[r][c][3] = 0.0f;
@@ -389,6 +477,53 @@ static void sample_full(const rr_image *img, const rr_sampler *s, const float co
   }
 }
 
+/* reference texture_sampling.cpp:186-250 (sample_cube_wrapped) face selection: six tests in the
+ * order +X, -X, +Y, -Y, +Z, -Z, each overriding the previous ones, so ties go to Z, then Y. The
+ * face coordinate is 0.5 * (sc / |ma| + 1). When no test holds (a NaN component) the reference
+ * reads uninitialised values; here face 0 with sc = tc = 0, |ma| = 1 (parity unpinned there). */
+static void cube_face(float x, float y, float z, uint32_t *face, float *u, float *v)
+{
+  const float ax = fabsf(x), ay = fabsf(y), az = fabsf(z);
+  const int px = x > 0.0f, py = y > 0.0f, pz = z > 0.0f;
+  float axis = 1.0f, sc = 0.0f, tc = 0.0f;
+  uint32_t f = 0;
+  if(px && ax >= ay && ax >= az) { axis = ax; sc = -z; tc = -y; f = 0; }
+  if(!px && ax >= ay && ax >= az) { axis = ax; sc = z; tc = -y; f = 1; }
+  if(py && ay >= ax && ay >= az) { axis = ay; sc = x; tc = z; f = 2; }
+  if(!py && ay >= ax && ay >= az) { axis = ay; sc = x; tc = -z; f = 3; }
+  if(pz && az >= ax && az >= ay) { axis = az; sc = x; tc = -y; f = 4; }
+  if(!pz && az >= ax && az >= ay) { axis = az; sc = -x; tc = -y; f = 5; }
+  *face = f;
+  *u = 0.5f * (sc / axis + 1.0f);
+  *v = 0.5f * (tc / axis + 1.0f);
+}
+
+void rr_sample_cube_wrapped(float x, float y, float z, const rr_image *tex, float out[4])
+{
+  uint32_t face;
+  float u, v;
+  cube_face(x, y, z, &face, &u, &v);
+  rr_sample_tex_wrapped(u, v, tex, rr_subresource_offset(tex, 0, face), out);
+}
+
+/* Cube sampling through the full sampler: the face is the array layer, no derivative LOD (the
+ * level follows the sampler's bias and clamps). With REPEAT + LINEAR and a single level it is
+ * sample_cube_wrapped exactly. */
+static void sample_cube_full(const rr_image *img, const rr_sampler *s, float x, float y, float z, float out[4])
+{
+  uint32_t face;
+  float c[3], zero[2] = {0.0f, 0.0f};
+  cube_face(x, y, z, &face, &c[0], &c[1]);
+  c[2] = (float)face;
+  sample_full(img, s, c, zero, zero, 0, out);
+}
+
+void rr_sample_cube(const rr_image *img, const rr_sampler *s, const float dir[3], float out[4])
+{
+  atomic_fetch_add(&tc_generation, 1);
+  sample_cube_full(img, s, dir[0], dir[1], dir[2], out);
+}
+
 /* ---------------------------------------------------------------------------------------- */
 /* Vertex fetch and the restated shaders                                                     */
 
@@ -557,6 +692,9 @@ static void fs_program(const rr_state *st, float pixdepth, const float b3[3][4],
       break;
     case RR_FS_TEX_ONLY:
       tex_uv(st, 0, b3, tri, 1, out);
+      break;
+    case RR_FS_CUBE_ENV:
+      sample_cube_full(st->tex[0], st->samp[0], interp(b, tri, 0, 0), interp(b, tri, 0, 1), interp(b, tri, 0, 2), out);
       break;
     case RR_FS_LAMBERT_TEX:
     case RR_FS_LAMBERT_TEX_A:

@@ -40,6 +40,7 @@ enum {
   RR_BF_SRC_ALPHA = 6,
   RR_BF_ONE_MINUS_SRC_ALPHA = 7,
   RR_BLEND_OP_ADD = 0,
+  RR_FMT_R8_UNORM = 9,
   RR_FMT_R8G8B8A8_UNORM = 37,
   RR_FMT_B8G8R8A8_UNORM = 44,
   RR_FMT_R16G16B16A16_SFLOAT = 97,
@@ -48,6 +49,8 @@ enum {
   RR_FMT_R32G32B32_SFLOAT = 106,
   RR_FMT_R32G32B32A32_SFLOAT = 109,
   RR_FMT_D32_SFLOAT = 126,
+  RR_FMT_BC2_UNORM_BLOCK = 135,
+  RR_FMT_BC3_UNORM_BLOCK = 137,
   RR_FILTER_NEAREST = 0,
   RR_FILTER_LINEAR = 1,
   RR_MIPMAP_NEAREST = 0,
@@ -66,13 +69,14 @@ enum {
   RR_FS_LAMBERT_TEX_A = 103,  /* as above with alpha = pc.alpha (push constant 112)  */
   RR_FS_MULTITAP8 = 104,      /* 8 array-texture taps accumulated in FP32             */
   RR_FS_TEX_ONLY = 105,       /* out = texture(tex, in1.xy)                           */
+  RR_FS_CUBE_ENV = 106,       /* out = texture(samplerCube, in0.xyz)                  */
 };
 
 typedef struct rr_image {
   uint8_t *pixels;          /* linear rows; mips packed within a layer, then layers     */
   uint32_t width, height;
   uint32_t mip_levels, array_layers;
-  uint32_t bpp;             /* bytes per pixel: 4 (RGBA8/BGRA8/D32) or 8 (RGBA16F)      */
+  uint32_t bpp;             /* bytes per pixel: 4 (RGBA8/BGRA8/D32), 8 (RGBA16F), 1 (R8, BC2, BC3) */
   uint32_t format;
 } rr_image;
 
@@ -131,6 +135,9 @@ void rr_to_window(uint32_t w, uint32_t h, const float pos[4], int32_t out[2]);
 void rr_sample(const rr_image *img, const rr_sampler *samp, const float coord[3],
                const float ddx[2], const float ddy[2], int use_lod, float out[4]);
 void rr_sample_tex_wrapped(float u, float v, const rr_image *tex, uint64_t byte_offs, float out[4]);
+void rr_sample_cube_wrapped(float x, float y, float z, const rr_image *tex, float out[4]);
+void rr_sample_cube(const rr_image *img, const rr_sampler *samp, const float dir[3], float out[4]);
+void rr_decode_bc_block(uint32_t format, const uint8_t block[16], uint8_t out[64]);
 float rr_log2(float x);
 uint16_t rr_f32_to_f16(float f);
 float rr_f16_to_f32(uint16_t h);

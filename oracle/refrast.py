@@ -64,6 +64,9 @@ def lib():
         l.rr_sample.argtypes = [C.POINTER(Image), C.POINTER(Sampler), C.POINTER(f32), C.POINTER(f32), C.POINTER(f32),
                                 C.c_int, C.POINTER(f32)]
         l.rr_sample_tex_wrapped.argtypes = [f32, f32, C.POINTER(Image), u64, C.POINTER(f32)]
+        l.rr_sample_cube_wrapped.argtypes = [f32, f32, f32, C.POINTER(Image), C.POINTER(f32)]
+        l.rr_sample_cube.argtypes = [C.POINTER(Image), C.POINTER(Sampler), C.POINTER(f32), C.POINTER(f32)]
+        l.rr_decode_bc_block.argtypes = [u32, C.POINTER(C.c_uint8), C.POINTER(C.c_uint8)]
         l.rr_log2.argtypes = [f32]
         l.rr_log2.restype = f32
         l.rr_f32_to_f16.argtypes = [f32]
@@ -76,6 +79,14 @@ def lib():
     return _lib
 
 
+def decode_bc_block(fmt, block):
+    """16 RGBA8 texels (row-major, shape (16, 4)) of one 16-byte BC2/BC3 block."""
+    b = (C.c_uint8 * 16).from_buffer_copy(bytes(block))
+    out = (C.c_uint8 * 64)()
+    lib().rr_decode_bc_block(fmt, b, out)
+    return np.frombuffer(bytes(out), dtype=np.uint8).reshape(16, 4)
+
+
 def make_image(arr, width, height, mips=1, layers=1, bpp=4, fmt=37):
     """Wraps a numpy uint8 buffer (kept alive by the caller) as an rr_image."""
     return Image(pixels=arr.ctypes.data, width=width, height=height, mip_levels=mips, array_layers=layers, bpp=bpp,
@@ -85,7 +96,7 @@ def make_image(arr, width, height, mips=1, layers=1, bpp=4, fmt=37):
 def render(scene, threads=0, draws=None, time_only=False):
     """Renders `scene`; returns dict(color, depth, counters, seconds). threads=0: serial API-order
     parity mode; threads>0: the reference-structured threaded mode (timing only)."""
-    from visor_amd import shaders  # program ids of the restated shaders
+    from visor_amd import shaders, vk  # program ids of the restated shaders, texel sizes
     L = lib()
     s = scene
     bpp = s.color_bpp
@@ -99,7 +110,7 @@ def render(scene, threads=0, draws=None, time_only=False):
     for t in s.textures:
         a = np.frombuffer(t.data, dtype=np.uint8)
         tex_arrays.append(a)
-        tex_imgs.append(make_image(a, t.width, t.height, t.mips, t.layers))
+        tex_imgs.append(make_image(a, t.width, t.height, t.mips, t.layers, bpp=vk.texel_bytes(t.format), fmt=t.format))
         samplers.append(Sampler(mag_filter=t.mag_filter, min_filter=t.min_filter, mipmap_mode=t.mipmap_mode,
                                 address_u=t.address, address_v=t.address, mip_lod_bias=t.lod_bias, min_lod=t.min_lod,
                                 max_lod=t.max_lod))

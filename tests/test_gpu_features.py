@@ -1,7 +1,8 @@
 """Fixed-function and sampler features on the GPU, each against the oracle: every depth compare op
 with and without depth writes, every cull mode x front face, all blend-factor pairs the reference
 implements, triangle strips, 16-bit indices, a loading (not clearing) second render pass, clear
-values outside [0, 1], and sampler modes. Small targets; bit-exact colour and depth."""
+values outside [0, 1], sampler modes, BC2/BC3 textures and cube maps. Small targets; bit-exact
+colour and depth."""
 import dataclasses
 
 import numpy as np
@@ -10,7 +11,8 @@ import pytest
 from oracle import refrast
 from visor_amd import render, scenes
 from visor_amd.vk import (ADDR_CLAMP, ADDR_REPEAT, BF, CMP, CULL_BACK, CULL_FRONT, CULL_NONE, FILTER_LINEAR,
-                          FILTER_NEAREST, FRONT_CCW, FRONT_CW, INDEX_U16, MIPMAP_LINEAR, MIPMAP_NEAREST, TOPO_STRIP)
+                          FILTER_NEAREST, FMT, FRONT_CCW, FRONT_CW, INDEX_U16, MIPMAP_LINEAR, MIPMAP_NEAREST,
+                          TOPO_STRIP)
 
 pytestmark = pytest.mark.gpu
 
@@ -158,3 +160,49 @@ def test_sampler_modes(ctx, k):
     s = scenes.many_draws(ndraws=10, width=160, height=120, tex_size=64, max_tris=1024, seed=21)
     s.textures = [dataclasses.replace(t, **SAMPLERS[k]) for t in s.textures]
     _check(ctx, s)
+
+
+BC_FORMATS = ["BC2_UNORM_BLOCK", "BC3_UNORM_BLOCK"]
+
+
+def _bc_draws(fmt, k, seed):
+    s = scenes.many_draws(ndraws=10, width=160, height=120, tex_size=64, max_tris=1024, seed=21)
+    rng = scenes.Rng(seed)
+    s.textures = [dataclasses.replace(t, format=FMT[fmt], data=scenes.bc_texture_data(rng, t.width, t.mips, t.layers),
+                                      **SAMPLERS[k]) for t in s.textures]
+    return s
+
+
+@pytest.mark.parametrize("fmt", BC_FORMATS)
+@pytest.mark.parametrize("k", [0, 3])
+def test_bc_textures(ctx, fmt, k):
+    """Random BC2/BC3 blocks (both colour modes, both alpha modes) with mips, sampled nearest or
+    trilinear."""
+    _check(ctx, _bc_draws(fmt, k, 40 + k))
+
+
+@pytest.mark.parametrize("fmt", ["R8G8B8A8_UNORM"] + BC_FORMATS)
+@pytest.mark.parametrize("batched", [True, False])
+def test_cube_env_sphere(ctx, fmt, batched):
+    """samplerCube lookups along the interpolated normal of a sphere: every face and the edges
+    between faces."""
+    _check(ctx, scenes.env_sphere(fmt=FMT[fmt]), batched=batched)
+
+
+def test_bc_texture_rewritten_between_submits(ctx):
+    """A BC texture rewritten by a buffer-to-image copy between two submits: the second render
+    samples the new blocks."""
+    s = _bc_draws("BC3_UNORM_BLOCK", 0, 50)
+    r = render.SceneRenderer(ctx, s)
+    ctx.set_mode(batched=True, tile=8)
+    r.render()
+    first = r.read_color()
+    assert np.array_equal(first, refrast.render(s)["color"])
+    rng = scenes.Rng(51)
+    for i, t in enumerate(s.textures):
+        s.textures[i] = dataclasses.replace(t, data=scenes.bc_texture_data(rng, t.width, t.mips, t.layers))
+        ctx.upload_texture(r.textures[i][0], s.textures[i])
+    r.render()
+    second = r.read_color()
+    assert not np.array_equal(first, second)
+    assert np.array_equal(second, refrast.render(s)["color"])

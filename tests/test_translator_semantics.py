@@ -48,36 +48,69 @@ def _translate(L, spv):
     return out.value.decode()
 
 
-def run_fs(L, tmp_path, spv, inputs, push=b""):
-    """Runs the fragment program on vec4 inputs (locations 0..n-1); returns its 4 outputs."""
+def run_fs_rows(L, tmp_path, spv, rows, push=b"", tex=None):
+    """Runs the fragment program once per row of vec4 inputs (locations 0..n-1); returns the
+    outputs, shape (len(rows), 4). `tex` = dict(data, width, height, layers, fmt, bpp, filter)
+    binds a host-memory texture (one level, REPEAT) at binding 0, read through its RGBA8 transcode
+    as the executor does (fmt 0 plain, 1 BC2, 2 BC3)."""
     code = _translate(L, spv)
     vary = []
-    for v in inputs:
-        for x in v:
-            vary += [float(np.float32(x)), 0.0, 0.0]
+    for inputs in rows:
+        for v in inputs:
+            for x in v:
+                vary += [float(np.float32(x)), 0.0, 0.0]
+    nv = max(1, len(vary) // max(1, len(rows)))
     vary = vary or [0.0]
     push = bytes(push) + b"\0" * (128 - len(push))
+    tex_init = ""
+    if tex is not None:
+        # the executor's transcode (texel_decode.h) fills the RGBA8 copy the samplers read
+        t = tex
+        tex_init = (r"""
+  static const unsigned char texdata[] = {""" + ",".join(str(b) for b in t["data"]) + r"""};
+  static uint32_t shadow[%d * %d * %d];
+  const uint32_t W = %d, H = %d, L = %d, BPP = %d, FMT = %d;
+  for(uint32_t l = 0; l < L; l++)
+    for(uint32_t y = 0; y < H; y++)
+      for(uint32_t x = 0; x < W; x++)
+        shadow[(l * H + y) * W + x] = va_texel_bytes(texdata, sizeof(texdata), (uint64_t)l * W * H * BPP, W, BPP, FMT, x, y);
+  VaTex &t = d.tex[0];
+  t.base = (unsigned long long)(uintptr_t)shadow;
+  t.width = W; t.height = H; t.mips = 1; t.layers = L; t.bpp = 4;
+  t.slice_bytes = (unsigned long long)W * H * 4;
+  t.mag_filter = t.min_filter = %d; t.mipmap_mode = 0; t.addr_u = t.addr_v = 0;
+  t.lod_bias = 0.0f; t.min_lod = 0.0f; t.max_lod = 0.0f;
+""" % (t["width"], t["height"], t["layers"], t["width"], t["height"], t["layers"], t["bpp"], t["fmt"],
+       t.get("filter", 1)))
     src = (SHIM + open(os.path.join(CSRC, "device_abi.h")).read() + "\n" +
+           open(os.path.join(CSRC, "texel_decode.h")).read() + "\n" +
            open(os.path.join(CSRC, "pipeline_prelude.hip")).read() + "\n" + code + r"""
 int main(void) {
   static VaDraw d;
   memset(&d, 0, sizeof(d));
   static const unsigned char push[128] = {""" + ",".join(str(b) for b in push) + r"""};
-  memcpy(d.push, push, 128);
+  memcpy(d.push, push, 128);""" + tex_init + r"""
   static const float vary[] = {""" + ",".join(repr(x) + "f" for x in vary) + r"""};
   const float b[3][4] = {{1.0f, 0.0f, 0.0f, 0.0f}, {1.0f, 0.0f, 0.0f, 0.0f}, {1.0f, 0.0f, 0.0f, 0.0f}};
-  float out[4];
-  va_shader(d, vary, b, out);
-  for(int i = 0; i < 4; i++) printf("%08x\n", __float_as_uint(out[i]));
+  for(int r = 0; r < %d; r++) {
+    float out[4];
+    va_shader(d, vary + r * %d, b, out);
+    for(int i = 0; i < 4; i++) printf("%%08x\n", __float_as_uint(out[i]));
+  }
   return 0;
 }
-""")
+""" % (max(1, len(rows)), nv))
     path = tmp_path / "fs.cpp"
     path.write_text(src)
     exe = tmp_path / "fs"
     subprocess.run(["g++", "-std=c++17", "-O1", "-ffp-contract=off", "-w", str(path), "-o", str(exe)], check=True)
     lines = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split()
-    return np.array([int(x, 16) for x in lines], dtype=np.uint32).view(np.float32)
+    return np.array([int(x, 16) for x in lines], dtype=np.uint32).view(np.float32).reshape(-1, 4)
+
+
+def run_fs(L, tmp_path, spv, inputs, push=b""):
+    """Runs the fragment program on vec4 inputs (locations 0..n-1); returns its 4 outputs."""
+    return run_fs_rows(L, tmp_path, spv, [inputs], push)[0]
 
 
 class FS:
@@ -395,3 +428,71 @@ def test_matrix_inverse(lib, tmp_path):
     got = run_fs(lib, tmp_path, p.finish(out), [A, B], push=M.T.tobytes())
     expect = np.linalg.inv(M.astype(np.float64)) @ np.float64(A)
     assert np.allclose(got, expect, rtol=1e-5, atol=1e-5)
+
+
+# --- texture sampling of the device prelude, run on the host, against the oracle --------------------
+
+def _cube_fs(cube=True):
+    p = FS(1)
+    m = p.m
+    v3 = m.vec(3)
+    tex = m.sampler(0, 0, cube=cube)
+    d = m.op("VectorShuffle", v3, p.vals[0], p.vals[0], 0, 1, 2)
+    if not cube:
+        d = m.op("VectorShuffle", m.vec(2), p.vals[0], p.vals[0], 0, 1)
+    t = m.op("ImageSampleImplicitLod", p.v4, m.load(m.sampled_image(cube=cube), tex), d)
+    return p.finish([m.op("CompositeExtract", p.f, t, k) for k in range(4)])
+
+
+@pytest.mark.parametrize("fmt", ["R8G8B8A8_UNORM", "BC2_UNORM_BLOCK", "BC3_UNORM_BLOCK"])
+def test_prelude_cube_sampling_matches_oracle(lib, tmp_path, fmt):
+    """Translated samplerCube lookup, the device prelude's face selection and bilinear filter, and
+    the executor's texel transcode (BC decode), compiled for the host: bit-exact with the oracle's
+    sample_cube_wrapped (ties, axes and random directions)."""
+    from oracle import refrast
+    from visor_amd.vk import FMT
+    W = 16
+    code = FMT[fmt]
+    bpp = 1 if "BC" in fmt else 4
+    rng = np.random.default_rng(17)
+    data = rng.integers(0, 256, 6 * W * W * bpp, dtype=np.uint8)
+    dirs = rng.normal(size=(150, 3)).astype(np.float32).tolist()
+    dirs += [[1, 1, 1], [1, 1, 0], [-1, -1, 0], [1, 0, -1], [0, 0, 1], [0, -1, 0], [-1, 0, 0], [0.5, -0.5, 0.5]]
+    got = run_fs_rows(lib, tmp_path, _cube_fs(), [[d + [0.0]] for d in dirs],
+                      tex=dict(data=data.tobytes(), width=W, height=W, layers=6, bpp=bpp,
+                               fmt={37: 0, 135: 1, 137: 2}[code]))
+    img = refrast.make_image(data, W, W, 1, 6, bpp=bpp, fmt=code)
+    out = (C.c_float * 4)()
+    L = refrast.lib()
+    for k, d in enumerate(dirs):
+        L.rr_sample_cube_wrapped(float(np.float32(d[0])), float(np.float32(d[1])), float(np.float32(d[2])),
+                                 C.byref(img), out)
+        want = np.frombuffer(bytes(out), np.float32)
+        assert np.array_equal(got[k], want), (fmt, d, got[k], want)
+
+
+@pytest.mark.parametrize("fmt", ["BC2_UNORM_BLOCK", "BC3_UNORM_BLOCK"])
+@pytest.mark.parametrize("filt", [0, 1])
+def test_prelude_bc_2d_sampling_matches_oracle(lib, tmp_path, fmt, filt):
+    """2D BC2/BC3 sampling (texel transcode, then the device prelude's nearest or bilinear filter,
+    repeat) on the host against the oracle's sampler, including coordinates outside [0, 1)."""
+    from oracle import refrast
+    from visor_amd.vk import FMT
+    W, H = 20, 12
+    code = FMT[fmt]
+    rng = np.random.default_rng(23)
+    data = rng.integers(0, 256, W * H, dtype=np.uint8)
+    uvs = rng.uniform(-1.5, 2.5, size=(200, 2)).astype(np.float32).tolist()
+    got = run_fs_rows(lib, tmp_path, _cube_fs(cube=False), [[uv + [0.0, 0.0]] for uv in uvs],
+                      tex=dict(data=data.tobytes(), width=W, height=H, layers=1, bpp=1,
+                               fmt={135: 1, 137: 2}[code], filter=filt))
+    img = refrast.make_image(data, W, H, 1, 1, bpp=1, fmt=code)
+    smp = refrast.Sampler(mag_filter=filt, min_filter=filt, mipmap_mode=0, address_u=0, address_v=0,
+                          mip_lod_bias=0.0, min_lod=0.0, max_lod=0.0)
+    out = (C.c_float * 4)()
+    L = refrast.lib()
+    for k, uv in enumerate(uvs):
+        L.rr_sample(C.byref(img), C.byref(smp), (C.c_float * 3)(uv[0], uv[1], 0.0), (C.c_float * 2)(),
+                    (C.c_float * 2)(), 0, out)
+        want = np.frombuffer(bytes(out), np.float32)
+        assert np.array_equal(got[k], want), (fmt, uv, got[k], want)

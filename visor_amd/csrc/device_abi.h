@@ -46,7 +46,7 @@ typedef struct VaTex {
   unsigned long long slice_bytes;   /* bytes per array layer (all mips) */
   unsigned int width, height, mips, layers;
   unsigned int mag_filter, min_filter, mipmap_mode, addr_u;
-  unsigned int addr_v, bpp, pad0, pad1;
+  unsigned int addr_v, bpp, pad0, pad1;  /* texels are 4-byte RGBA (bpp 4) */
   float lod_bias, min_lod, max_lod, pad2;
   unsigned int level_off[VA_MAX_LEVELS]; /* byte offset of each mip inside a layer */
 } VaTex;

@@ -14,6 +14,7 @@
 
 #include "internal.h"
 #include "runtime_kernels.h"
+#include "texel_decode.h"
 
 namespace visor
 {
@@ -165,26 +166,81 @@ static uint64_t SubresourceOffset(const VkImage_T *img, uint32_t mip, uint32_t l
   return offs;
 }
 
-static void FillTex(VaTex &t, const VkImage_T *img, const VkSampler_T *s)
+// Images whose texels are not 4-byte RGBA are sampled through an RGBA8 shadow holding, per
+// texel, the 4 bytes the reference's fetch returns (BC2/BC3 decoded, R8 and RGBA16F as the
+// reference reads them): the pipelines' samplers then read one aligned word per texel.
+static bool NeedsShadow(const VkImage_T *img)
 {
-  memset(&t, 0, sizeof(t));
-  t.base = (unsigned long long)(uintptr_t)img->devPixels();
-  t.width = img->extent.width;
-  t.height = img->extent.height;
-  t.mips = std::max(1u, std::min<uint32_t>(img->mipLevels, VA_MAX_LEVELS));
-  t.layers = std::max(1u, img->arrayLayers);
-  t.bpp = img->bytesPerPixel;
+  return img->bytesPerPixel != 4 || img->format == VK_FORMAT_BC2_UNORM_BLOCK ||
+         img->format == VK_FORMAT_BC3_UNORM_BLOCK;
+}
+
+static uint32_t ShadowLevels(const VkImage_T *img) { return std::max(1u, std::min<uint32_t>(img->mipLevels, VA_MAX_LEVELS)); }
+
+// Layout of a linear image: levels packed per layer, layers back to back (precompiled.cpp:3-36).
+static void LevelLayout(const VkImage_T *img, uint32_t bpp, uint32_t levels, uint64_t *level_off, uint64_t &slice)
+{
   uint64_t off = 0;
-  uint32_t w = t.width, h = t.height;
-  for(uint32_t m = 0; m < img->mipLevels; m++)
+  uint32_t w = img->extent.width, h = img->extent.height;
+  for(uint32_t m = 0; m < levels; m++)
   {
-    if(m < VA_MAX_LEVELS)
-      t.level_off[m] = (unsigned)off;
-    off += (uint64_t)w * h * img->bytesPerPixel;
+    if(level_off && m < VA_MAX_LEVELS)
+      level_off[m] = off;
+    off += (uint64_t)w * h * bpp;
     w = std::max(1u, w >> 1);
     h = std::max(1u, h >> 1);
   }
-  t.slice_bytes = off;
+  slice = off;
+}
+
+static bool EnsureShadow(VkImage_T *img, std::string &err)
+{
+  if(img->shadow)
+    return true;
+  uint64_t slice = 0;
+  LevelLayout(img, 4, ShadowLevels(img), nullptr, slice);
+  const uint64_t bytes = slice * std::max(1u, img->arrayLayers);
+  if(bytes == 0 || hipMalloc((void **)&img->shadow, bytes) != hipSuccess)
+  {
+    img->shadow = nullptr;
+    err = "out of device memory for a sampled image's RGBA8 copy";
+    return false;
+  }
+  return true;
+}
+
+static hipError_t TranscodeAsync(const VkImage_T *img, hipStream_t stream)
+{
+  VaTranscode t;
+  memset(&t, 0, sizeof(t));
+  t.width = img->extent.width;
+  t.height = img->extent.height;
+  t.mips = ShadowLevels(img);
+  t.layers = std::max(1u, img->arrayLayers);
+  t.bpp = img->bytesPerPixel;
+  t.fmt = img->format == VK_FORMAT_BC2_UNORM_BLOCK ? VA_TEXFMT_BC2
+          : img->format == VK_FORMAT_BC3_UNORM_BLOCK ? VA_TEXFMT_BC3 : VA_TEXFMT_PLAIN;
+  LevelLayout(img, img->bytesPerPixel, img->mipLevels, t.src_level_off, t.src_slice);
+  LevelLayout(img, 4, t.mips, t.dst_level_off, t.dst_slice);
+  t.src_bytes = t.src_slice * t.layers;
+  return va_transcode_async(img->devPixels(), img->shadow, &t, stream);
+}
+
+static void FillTex(VaTex &t, const VkImage_T *img, const VkSampler_T *s)
+{
+  memset(&t, 0, sizeof(t));
+  const bool sh = NeedsShadow(img);
+  t.base = (unsigned long long)(uintptr_t)(sh ? img->shadow : img->devPixels());
+  t.width = img->extent.width;
+  t.height = img->extent.height;
+  t.mips = ShadowLevels(img);
+  t.layers = std::max(1u, img->arrayLayers);
+  t.bpp = 4;
+  uint64_t level_off[VA_MAX_LEVELS], slice = 0;
+  LevelLayout(img, 4, sh ? t.mips : img->mipLevels, level_off, slice);
+  for(uint32_t m = 0; m < t.mips; m++)
+    t.level_off[m] = (unsigned)level_off[m];
+  t.slice_bytes = slice;
   if(s)
   {
     const VkSamplerCreateInfo &i = s->info;
@@ -449,6 +505,12 @@ bool Queue::Decode(const VkCommandBuffer_T *cb, Submission &sub, std::string &er
             {
               err = "sampled image has no memory bound";
               return false;
+            }
+            if(NeedsShadow(img))
+            {
+              if(!EnsureShadow(img, err))
+                return false;
+              dd.shadowed[r.binding] = img;
             }
             FillTex(v.tex[r.binding], img, b.imageInfo.sampler);
             AddMem(sub.uploads, img->mem);
@@ -761,6 +823,18 @@ VkResult Queue::RunPass(const std::vector<Op *> &passes, bool fused_clear, ExecS
   for(int i = 0; i < 8; i++)
     if(!ev[i])
       HIPCHK(hipEventCreate(&ev[i]));
+
+  // sampled images read through an RGBA8 shadow get it refreshed from their current contents
+  {
+    std::vector<const VkImage_T *> shadows;
+    for(Op *p : passes)
+      for(const DrawDesc &dd : p->draws)
+        for(const VkImage_T *im : dd.shadowed)
+          if(im && std::find(shadows.begin(), shadows.end(), im) == shadows.end())
+            shadows.push_back(im);
+    for(const VkImage_T *im : shadows)
+      HIPCHK(TranscodeAsync(im, stream));
+  }
 
   // the per-pass tables go up only when they changed (a re-submitted command buffer reuses them)
   if(d_frames != up_frames_ptr || frames.size() != up_frames.size() ||

@@ -469,7 +469,12 @@ VKAPI_ATTR VkResult VKAPI_CALL vkCreateImage(VkDevice, const VkImageCreateInfo *
   return VK_SUCCESS;
 }
 
-VKAPI_ATTR void VKAPI_CALL vkDestroyImage(VkDevice, VkImage image, const VkAllocationCallbacks *) { delete image; }
+VKAPI_ATTR void VKAPI_CALL vkDestroyImage(VkDevice, VkImage image, const VkAllocationCallbacks *)
+{
+  if(image && image->shadow)
+    (void)hipFree(image->shadow);  // waits for work still reading it
+  delete image;
+}
 
 VKAPI_ATTR VkResult VKAPI_CALL vkBindImageMemory(VkDevice, VkImage image, VkDeviceMemory memory, VkDeviceSize offset)
 {

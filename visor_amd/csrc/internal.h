@@ -119,6 +119,9 @@ struct VkImage_T
   uint32_t bytesPerPixel = 4;
   VkDeviceMemory_T *mem = nullptr;
   VkDeviceSize offset = 0;
+  // RGBA8 copy that the samplers read when the texels are not 4-byte RGBA (BC2/BC3, R8,
+  // RGBA16F): refreshed by a transcode launch before every pass that samples the image
+  uint8_t *shadow = nullptr;
   uint8_t *devPixels() const { return mem ? mem->dev + offset : nullptr; }
 };
 
@@ -287,6 +290,7 @@ struct DrawDesc
   uint32_t frame;          // index into the pass list
   VaDraw d;                // everything but tri_base / frame / slot
   VkPipeline_T *pipeline;
+  VkImage_T *shadowed[VA_MAX_TEX] = {};  // sampled images read through their RGBA8 shadow
 };
 
 struct CopyDesc

@@ -273,6 +273,37 @@ VA_INLINE void va_tex_lod(const VaDraw &d, u32 slot, df u, df v, df layer, df lo
   va_sample(d.tex[slot], u.c, v.c, layer.c, 0.0f, 0.0f, 0.0f, 0.0f, 2, lod.c, out);
 }
 
+// Cube maps (reference texture_sampling.cpp:186-250): the face tests run in the order +X, -X,
+// +Y, -Y, +Z, -Z, each overriding the previous ones (ties go to Z, then Y); the face coordinate
+// is 0.5 * (sc / |ma| + 1). The face is then sampled as array layer `face`, without derivative
+// LOD. A NaN direction, where no test holds, samples face 0 at (0.5, 0.5).
+VA_INLINE void va_cube_face(float x, float y, float z, float &u, float &v, float &face)
+{
+  const float ax = fabsf(x), ay = fabsf(y), az = fabsf(z);
+  const bool px = x > 0.0f, py = y > 0.0f, pz = z > 0.0f;
+  float axis = 1.0f, sc = 0.0f, tc = 0.0f, f = 0.0f;
+  if(px && ax >= ay && ax >= az) { axis = ax; sc = -z; tc = -y; f = 0.0f; }
+  if(!px && ax >= ay && ax >= az) { axis = ax; sc = z; tc = -y; f = 1.0f; }
+  if(py && ay >= ax && ay >= az) { axis = ay; sc = x; tc = z; f = 2.0f; }
+  if(!py && ay >= ax && ay >= az) { axis = ay; sc = x; tc = -z; f = 3.0f; }
+  if(pz && az >= ax && az >= ay) { axis = az; sc = x; tc = -y; f = 4.0f; }
+  if(!pz && az >= ax && az >= ay) { axis = az; sc = -x; tc = -y; f = 5.0f; }
+  u = 0.5f * (sc / axis + 1.0f);
+  v = 0.5f * (tc / axis + 1.0f);
+  face = f;
+}
+VA_INLINE void va_tex_cube(const VaDraw &d, u32 slot, float x, float y, float z, int use_lod, float lod,
+                           float out[4])
+{
+  float u, v, f;
+  va_cube_face(x, y, z, u, v, f);
+  va_sample(d.tex[slot], u, v, f, 0.0f, 0.0f, 0.0f, 0.0f, use_lod, lod, out);
+}
+VA_INLINE void va_tex_cube(const VaDraw &d, u32 slot, df x, df y, df z, int use_lod, df lod, float out[4])
+{
+  va_tex_cube(d, slot, x.c, y.c, z.c, use_lod, lod.c, out);
+}
+
 // Interpolated fragment input (reference spirv_compile.cpp:2159-2240 + CreateDot :629-643):
 // (((b0*a0) + b1*a1) + b2*a2) + b3*0 with b3 = 0.
 VA_INLINE float va_interp(const float *b, const float *a)

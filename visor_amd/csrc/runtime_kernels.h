@@ -24,9 +24,23 @@ hipError_t va_fill_u64_async(void *p, uint64_t count, uint32_t lo, uint32_t hi, 
 }
 #endif
 
+/* Texel transcode: writes, for every texel of every level and layer, the 4 bytes the reference's
+ * texture fetch returns (texel_decode.h) into an RGBA8 image of the same extent, levels and
+ * layers (levels packed per layer, layers back to back). */
+#define VA_TRANSCODE_MAX_LEVELS 16
+typedef struct VaTranscode {
+  uint64_t src_bytes;   /* whole source image */
+  uint64_t src_slice;   /* source bytes per layer */
+  uint64_t dst_slice;   /* shadow bytes per layer */
+  uint32_t width, height, mips, layers, bpp, fmt;
+  uint64_t src_level_off[VA_TRANSCODE_MAX_LEVELS];
+  uint64_t dst_level_off[VA_TRANSCODE_MAX_LEVELS];
+} VaTranscode;
+
 #ifdef __cplusplus
 extern "C" {
 #endif
+hipError_t va_transcode_async(const uint8_t *src, uint8_t *dst, const VaTranscode *t, hipStream_t stream);
 hipError_t va_stamp_async(uint64_t *out, hipStream_t stream);
 hipError_t va_reduce_cols_async(const uint64_t *part, uint32_t rows, uint32_t cols, uint64_t *out, hipStream_t stream);
 #ifdef __cplusplus

@@ -6,6 +6,7 @@
 #include <hip/hip_runtime.h>
 
 #include "runtime_kernels.h"
+#include "texel_decode.h"
 
 typedef uint32_t u32;
 typedef uint64_t u64;
@@ -331,6 +332,37 @@ extern "C" hipError_t va_stamp_async(uint64_t *out, hipStream_t stream)
 {
   (void)hipGetLastError();  // report this launch's error only
   hipLaunchKernelGGL(va_stamp_kernel, dim3(1), dim3(64), 0, stream, out);
+  return hipGetLastError();
+}
+
+// One thread per shadow texel, grid-stride over the level; blockIdx.y = layer * mips + level.
+__global__ __launch_bounds__(256) void va_transcode_kernel(const uint8_t *__restrict__ src, uint8_t *__restrict__ dst,
+                                                           VaTranscode t)
+{
+  const uint32_t level = blockIdx.y % t.mips, layer = blockIdx.y / t.mips;
+  const uint32_t w = t.width >> level ? t.width >> level : 1u;
+  const uint32_t h = t.height >> level ? t.height >> level : 1u;
+  const uint64_t n = (uint64_t)w * h;
+  const uint64_t sub = t.src_slice * layer + t.src_level_off[level];
+  uint32_t *out = (uint32_t *)(dst + t.dst_slice * layer + t.dst_level_off[level]);
+  for(uint64_t i = (uint64_t)blockIdx.x * 256u + threadIdx.x; i < n; i += (uint64_t)gridDim.x * 256u)
+  {
+    const uint32_t y = (uint32_t)(i / w), x = (uint32_t)(i - (uint64_t)y * w);
+    out[i] = va_texel_bytes(src, t.src_bytes, sub, w, t.bpp, t.fmt, x, y);
+  }
+}
+
+extern "C" hipError_t va_transcode_async(const uint8_t *src, uint8_t *dst, const VaTranscode *t, hipStream_t stream)
+{
+  if(t->mips == 0u || t->mips > VA_TRANSCODE_MAX_LEVELS || t->layers == 0u || t->width == 0u || t->height == 0u ||
+     (uint64_t)t->mips * t->layers > 65535u)
+    return hipErrorInvalidValue;
+  const uint64_t texels = (uint64_t)t->width * t->height;
+  uint64_t bx = (texels + 255u) / 256u;
+  if(bx > 4096u)
+    bx = 4096u;
+  (void)hipGetLastError();
+  hipLaunchKernelGGL(va_transcode_kernel, dim3((unsigned)bx, t->mips * t->layers), dim3(256), 0, stream, src, dst, *t);
   return hipGetLastError();
 }
 

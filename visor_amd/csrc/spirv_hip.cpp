@@ -67,6 +67,8 @@ enum Glsl
   G_Cross = 68, G_Normalize = 69, G_Reflect = 71,
 };
 
+constexpr uint32_t kDimCube = 3;  // SPIR-V Dim operand of OpTypeImage
+
 enum Dec
 {
   D_Block = 2, D_RowMajor = 4, D_ColMajor = 5, D_ArrayStride = 6, D_MatrixStride = 7,
@@ -1505,9 +1507,15 @@ bool Translator::EmitInst(Frame &f, const Inst &in, uint32_t cur_label)
         return false;
       Val &r = result(1, 2);
       const Type &it = T(img.type);
+      // reference spirv_compile.cpp:896-899, 1290-1291, 1847-1881: a sampled image of a Cube
+      // image type samples by direction (x, y, z); every other one by (u, v)
+      const bool cube = it.dim == kDimCube;
+      if(coord.c.size() < (cube ? 3u : 2u))
+        return Fail("sampling coordinate has too few components");
       std::string layer = (it.arrayed && coord.c.size() > 2) ? coord.c[2] : "F(0.0f)";
       std::string t = "tx" + std::to_string(tmp_counter++);
       body << "  { float " << t << "[4];\n";
+      std::string lod_arg;
       if(in.op == OpImageSampleExplicitLod)
       {
         uint32_t mask = in.wc > 5 ? w[5] : 0;
@@ -1516,9 +1524,14 @@ bool Translator::EmitInst(Frame &f, const Inst &in, uint32_t cur_label)
         Val lod;
         if(!Get(f, w[6], lod))
           return false;
-        body << "  va_tex_lod(d, " << img.c[0] << ", " << coord.c[0] << ", " << coord.c[1] << ", " << layer << ", "
-             << lod.c[0] << ", " << t << ");\n";
+        lod_arg = lod.c[0];
       }
+      if(cube)
+        body << "  va_tex_cube(d, " << img.c[0] << ", " << coord.c[0] << ", " << coord.c[1] << ", " << coord.c[2] << ", "
+             << (lod_arg.empty() ? "0, F(0.0f)" : "2, " + lod_arg) << ", " << t << ");\n";
+      else if(!lod_arg.empty())
+        body << "  va_tex_lod(d, " << img.c[0] << ", " << coord.c[0] << ", " << coord.c[1] << ", " << layer << ", "
+             << lod_arg << ", " << t << ");\n";
       else
         body << "  va_tex(d, " << img.c[0] << ", " << coord.c[0] << ", " << coord.c[1] << ", " << layer << ", " << t << ");\n";
       for(uint32_t k = 0; k < r.c.size() && k < 4; k++)

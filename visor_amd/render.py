@@ -80,9 +80,10 @@ class Context:
         self.icd.vkDestroyBuffer(self.device, b, None)
         self.icd.vkFreeMemory(self.device, mem, None)
 
-    def image(self, w, h, fmt, mips=1, layers=1, host_visible=False):
+    def image(self, w, h, fmt, mips=1, layers=1, host_visible=False, cube=False):
         img = Handle()
-        ici = vk.ImageCreateInfo(imageType=1, format=fmt, extent=vk.Extent3D(width=w, height=h, depth=1), mipLevels=mips,
+        ici = vk.ImageCreateInfo(flags=vk.IMAGE_CREATE_CUBE_COMPATIBLE if cube else 0, imageType=1, format=fmt,
+                                 extent=vk.Extent3D(width=w, height=h, depth=1), mipLevels=mips,
                                  arrayLayers=layers, samples=1, tiling=1)
         self.icd.check(self.icd.vkCreateImage(self.device, C.byref(ici), None, C.byref(img)), "vkCreateImage")
         req = vk.MemoryRequirements()
@@ -90,7 +91,8 @@ class Context:
         mem = self.alloc(req.size, host_visible)
         self.icd.check(self.icd.vkBindImageMemory(self.device, img, mem, 0), "vkBindImageMemory")
         view = Handle()
-        vci = vk.ImageViewCreateInfo(image=img, viewType=5 if layers > 1 else 1, format=fmt,
+        view_type = vk.VIEW_TYPE_CUBE if cube else (vk.VIEW_TYPE_2D_ARRAY if layers > 1 else vk.VIEW_TYPE_2D)
+        vci = vk.ImageViewCreateInfo(image=img, viewType=view_type, format=fmt,
                                      subresourceRange=vk.ImageSubresourceRange(aspectMask=1, levelCount=mips,
                                                                                layerCount=layers))
         self.icd.check(self.icd.vkCreateImageView(self.device, C.byref(vci), None, C.byref(view)), "vkCreateImageView")
@@ -98,7 +100,18 @@ class Context:
 
     def texture(self, t):
         """Uploads a scenes.Texture (all levels and layers) with buffer-to-image copies."""
-        img, mem, view, size = self.image(t.width, t.height, vk.FMT["R8G8B8A8_UNORM"], t.mips, t.layers)
+        img, mem, view, size = self.image(t.width, t.height, t.format, t.mips, t.layers, cube=t.cube)
+        self.upload_texture(img, t)
+        samp = Handle()
+        sci = vk.SamplerCreateInfo(magFilter=t.mag_filter, minFilter=t.min_filter, mipmapMode=t.mipmap_mode,
+                                   addressModeU=t.address, addressModeV=t.address, addressModeW=t.address,
+                                   mipLodBias=t.lod_bias, minLod=t.min_lod, maxLod=t.max_lod)
+        self.icd.check(self.icd.vkCreateSampler(self.device, C.byref(sci), None, C.byref(samp)), "vkCreateSampler")
+        return img, mem, view, samp
+
+    def upload_texture(self, img, t):
+        """Copies t.data (every level and layer) into an existing image of t's shape."""
+        bpp = vk.texel_bytes(t.format)
         stage, smem = self.buffer(len(t.data), host_visible=True)
         C.memmove(self.map(smem, len(t.data)), t.data, len(t.data))
         cb = self.begin()
@@ -111,17 +124,11 @@ class Context:
                     bufferOffset=offs, imageSubresource=vk.ImageSubresourceLayers(aspectMask=1, mipLevel=m,
                                                                                   baseArrayLayer=layer, layerCount=1),
                     imageExtent=vk.Extent3D(width=w, height=h, depth=1)))
-                offs += w * h * 4
+                offs += w * h * bpp
         arr = (vk.BufferImageCopy * len(regions))(*regions)
         self.icd.vkCmdCopyBufferToImage(cb, stage, img, 7, len(regions), arr)
         self.end_submit(cb)
         self.free_buffer(stage, smem)
-        samp = Handle()
-        sci = vk.SamplerCreateInfo(magFilter=t.mag_filter, minFilter=t.min_filter, mipmapMode=t.mipmap_mode,
-                                   addressModeU=t.address, addressModeV=t.address, addressModeW=t.address,
-                                   mipLodBias=t.lod_bias, minLod=t.min_lod, maxLod=t.max_lod)
-        self.icd.check(self.icd.vkCreateSampler(self.device, C.byref(sci), None, C.byref(samp)), "vkCreateSampler")
-        return img, mem, view, samp
 
     # -- commands -------------------------------------------------------------------------------
     def begin(self):

@@ -56,7 +56,7 @@ class Texture:
     height: int
     mips: int
     layers: int
-    data: bytes  # RGBA8, layer-major, mips packed inside each layer
+    data: bytes  # texels in `format`, layer-major, mips packed inside each layer
     mag_filter: int = FILTER_LINEAR
     min_filter: int = FILTER_LINEAR
     mipmap_mode: int = MIPMAP_NEAREST
@@ -64,6 +64,8 @@ class Texture:
     lod_bias: float = 0.0
     min_lod: float = 0.0
     max_lod: float = 0.0
+    format: int = FMT["R8G8B8A8_UNORM"]  # R8G8B8A8_UNORM, BC2_UNORM_BLOCK or BC3_UNORM_BLOCK
+    cube: bool = False  # 6 layers viewed as a cube (+X, -X, +Y, -Y, +Z, -Z)
 
 
 @dataclass
@@ -296,6 +298,34 @@ def lit_sphere(stacks=250, slices=400, width=1920, height=1080, tex_size=1024, s
     draws = [Draw("lit", tri.size, 0, True, push_block(mvp, nmat), texture=0)]
     return Scene("lit_sphere", width, height, interleave_pnu(pos, nrm, uv), 32, VATTRS_PNU, pipes, draws,
                  index_data=tri.astype(np.uint32).tobytes(), textures=[tex])
+
+
+def bc_texture_data(rng, size, levels, layers=1):
+    """Random BC2/BC3 blocks for a size x size texture (size a multiple of 4): every 16-byte
+    pattern is a valid block, and random endpoints cover both colour-palette and both
+    alpha-palette modes. Level m holds (size >> m)^2 bytes, the linear layout's 1 byte per texel."""
+    out = []
+    for _ in range(layers):
+        for m in range(levels):
+            n = max(1, size >> m)
+            out.append(rng.bytes(n * n).tobytes())
+    return b"".join(out)
+
+
+def env_sphere(stacks=60, slices=90, width=256, height=192, tex_size=64, fmt=FMT["R8G8B8A8_UNORM"], seed=4):
+    """The perturbed sphere of lit_sphere, shaded by a cube-map lookup along the interpolated
+    normal (fs_cube_env): every face and the edges between faces are sampled. The cube texture is
+    RGBA8 noise or random BC2/BC3 blocks, one level, bilinear, repeat."""
+    s = lit_sphere(stacks, slices, width, height, tex_size, seed)
+    rng = Rng(seed + 100)
+    if fmt == FMT["R8G8B8A8_UNORM"]:
+        data = noise_texture(rng, tex_size, 1, layers=6, smooth=False)
+    else:
+        data = bc_texture_data(rng, tex_size, 1, layers=6)
+    s.textures = [Texture(tex_size, tex_size, 1, 6, data, format=fmt, cube=True)]
+    s.pipelines = {"lit": Pipeline("vs_mvp_normal_uv", "fs_cube_env", cull=CULL_BACK)}
+    s.name = "env_sphere"
+    return s
 
 
 def many_draws(ndraws=256, width=1920, height=1080, tex_size=512, seed=3, min_tris=16, max_tris=65536):

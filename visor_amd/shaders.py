@@ -155,6 +155,20 @@ def fs_tex_only():
     return m.to_bytes()
 
 
+def fs_cube_env():
+    """out = texture(samplerCube, in0.xyz): an environment lookup by the interpolated normal.
+    (RR_FS_CUBE_ENV)"""
+    m = Module()
+    v3, v4 = m.vec(3), m.vec(4)
+    nrm = m.input(v3, location=0)
+    tex = m.sampler(0, 0, cube=True)
+    o = m.output(v4, location=0)
+    m.begin_main()
+    m.store(o, m.op("ImageSampleImplicitLod", v4, m.load(m.sampled_image(cube=True), tex), m.load(v3, nrm)))
+    m.end_main("Fragment", "main", [nrm, o], frag=True)
+    return m.to_bytes()
+
+
 def fs_multitap8():
     """Sum of 8 weighted array-texture taps in FP32.  (RR_FS_MULTITAP8)"""
     m = Module()
@@ -182,7 +196,7 @@ def fs_multitap8():
 # oracle program ids (oracle/refrast.h)
 ORACLE_ID = {
     "vs_mvp_pos4_col": 1, "vs_mvp_normal_uv": 2, "fs_color": 101, "fs_lambert_tex": 102,
-    "fs_lambert_tex_a": 103, "fs_multitap8": 104, "fs_tex_only": 105,
+    "fs_lambert_tex_a": 103, "fs_multitap8": 104, "fs_tex_only": 105, "fs_cube_env": 106,
 }
 
 PROGRAMS = {
@@ -193,4 +207,5 @@ PROGRAMS = {
     "fs_lambert_tex_a": lambda: fs_lambert_tex(True),
     "fs_multitap8": fs_multitap8,
     "fs_tex_only": fs_tex_only,
+    "fs_cube_env": fs_cube_env,
 }

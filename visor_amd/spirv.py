@@ -105,12 +105,13 @@ class Module:
     def ptr(self, sc, t): return self._type(("ptr", sc, t), "TypePointer", [SC[sc], t])
     def func_type(self, ret, params=()):
         return self._type(("fn", ret, tuple(params)), "TypeFunction", [ret] + list(params))
-    def image2d(self, arrayed=False):
-        return self._type(("img", arrayed), "TypeImage",
-                          [self.float(), 1, 0, 1 if arrayed else 0, 0, 1, 0])
-    def sampled_image(self, arrayed=False):
-        img = self.image2d(arrayed)
-        return self._type(("simg", arrayed), "TypeSampledImage", [img])
+    def image2d(self, arrayed=False, cube=False):
+        """Float image type, Dim 2D (1) or Cube (3)."""
+        return self._type(("img", arrayed, cube), "TypeImage",
+                          [self.float(), 3 if cube else 1, 0, 1 if arrayed else 0, 0, 1, 0])
+    def sampled_image(self, arrayed=False, cube=False):
+        img = self.image2d(arrayed, cube)
+        return self._type(("simg", arrayed, cube), "TypeSampledImage", [img])
 
     # -- constants ---------------------------------------------------------------------------
     def const_f(self, v):
@@ -185,8 +186,8 @@ class Module:
                 return k
         return None
 
-    def sampler(self, set_, binding, arrayed=False):
-        v = self.variable("UniformConstant", self.sampled_image(arrayed))
+    def sampler(self, set_, binding, arrayed=False, cube=False):
+        v = self.variable("UniformConstant", self.sampled_image(arrayed, cube))
         self.decorate(v, "DescriptorSet", set_)
         self.decorate(v, "Binding", binding)
         return v

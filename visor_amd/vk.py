@@ -29,8 +29,16 @@ ST = dict(APPLICATION_INFO=0, INSTANCE_CREATE_INFO=1, DEVICE_QUEUE_CREATE_INFO=2
           FRAMEBUFFER_CREATE_INFO=37, RENDER_PASS_CREATE_INFO=38, COMMAND_POOL_CREATE_INFO=39,
           COMMAND_BUFFER_ALLOCATE_INFO=40, COMMAND_BUFFER_BEGIN_INFO=42, RENDER_PASS_BEGIN_INFO=43)
 
-FMT = dict(R8G8B8A8_UNORM=37, B8G8R8A8_UNORM=44, R16G16B16A16_SFLOAT=97, R32_SFLOAT=100,
-           R32G32_SFLOAT=103, R32G32B32_SFLOAT=106, R32G32B32A32_SFLOAT=109, D32_SFLOAT=126)
+FMT = dict(R8_UNORM=9, R8G8B8A8_UNORM=37, B8G8R8A8_UNORM=44, R16G16B16A16_SFLOAT=97, R32_SFLOAT=100,
+           R32G32_SFLOAT=103, R32G32B32_SFLOAT=106, R32G32B32A32_SFLOAT=109, D32_SFLOAT=126,
+           BC2_UNORM_BLOCK=135, BC3_UNORM_BLOCK=137)
+IMAGE_CREATE_CUBE_COMPATIBLE = 0x10
+VIEW_TYPE_2D, VIEW_TYPE_CUBE, VIEW_TYPE_2D_ARRAY = 1, 3, 5
+
+
+def texel_bytes(fmt):
+    """Bytes per pixel of the linear image layout (1 for R8 and the 4x4-block BC2/BC3 formats)."""
+    return {9: 1, 135: 1, 137: 1, 97: 8}.get(fmt, 4)
 TOPO_LIST, TOPO_STRIP = 3, 4
 CULL_NONE, CULL_FRONT, CULL_BACK = 0, 1, 2
 FRONT_CCW, FRONT_CW = 0, 1
