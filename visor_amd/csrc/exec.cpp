@@ -92,16 +92,13 @@ Queue::~Queue()
     qcv.notify_all();
     worker.join();
   }
-  void *bufs[] = {d_frames, d_draws, d_rec, d_count, d_offs, d_keys, d_vals, d_keys2, d_vals2, d_start, d_end, d_ctr, d_temp,
+  void *bufs[] = {d_frames, d_draws, d_rec, d_count, d_offs, d_keys, d_vals, d_keys2, d_vals2, d_start, d_end, d_temp,
                   d_blocks, d_part, d_stemp, d_einfo, d_stamps};
   for(void *b : bufs)
     if(b)
       hipFree(b);
   if(h_pinned)
     hipHostFree(h_pinned);
-  for(auto &e : ev)
-    if(e)
-      hipEventDestroy(e);
   for(auto &g : graph_cache)
     for(hipGraphExec_t x : g.exec)
       if(x)
@@ -813,16 +810,12 @@ VkResult Queue::RunPass(const std::vector<Op *> &passes, bool fused_clear, ExecS
      !Grow(&d_rec, c_rec, (size_t)stride * 4 * std::max(tri_total, 1u)) || !Grow(&d_count, c_count, 4ull * (tri_total + 1)) ||
      !Grow(&d_offs, c_offs, 4ull * (tri_total + 1)) || !Grow(&d_start, c_start, 4ull * (tile_total + 1)) ||
      !Grow(&d_einfo, c_einfo, 8ull * std::max(tri_total, 1u)) || !Grow(&d_stamps, c_stamps, 64) ||
-     !Grow(&d_ctr, c_ctr, sizeof(VaCounters)) ||
      !Grow(&d_blocks, c_blocks, 4ull * std::max<size_t>(nblocks, 1)) ||
-     !Grow(&d_part, c_part, 32ull * (std::max<size_t>(nblocks, tile_total) + 1)) ||
+     !Grow(&d_part, c_part, 32ull * ((size_t)nblocks + tile_total + 1)) ||
      !Grow(&d_temp, c_temp, 4 * va_scan_temp_words(tri_total + 1)))
     return VK_ERROR_OUT_OF_DEVICE_MEMORY;
-  if(!h_pinned)
-    HIPCHK(hipHostMalloc((void **)&h_pinned, 4096, hipHostMallocDefault));
-  for(int i = 0; i < 8; i++)
-    if(!ev[i])
-      HIPCHK(hipEventCreate(&ev[i]));
+  if(!h_pinned)  // written by the finish kernel, read by the host after the stream completes
+    HIPCHK(hipHostMalloc((void **)&h_pinned, 4096, hipHostMallocCoherent));
 
   // sampled images read through an RGBA8 shadow get it refreshed from their current contents
   {
@@ -875,12 +868,13 @@ VkResult Queue::RunPass(const std::vector<Op *> &passes, bool fused_clear, ExecS
     VaFrame *df = (VaFrame *)d_frames;
     uint32_t *rec = (uint32_t *)d_rec, *cnt = (uint32_t *)d_count, *offs = (uint32_t *)d_offs;
     uint32_t *bd = (uint32_t *)d_blocks, *total_dev = offs + tri_total;
-    VaCounters *ctr = (VaCounters *)d_ctr;
-    uint64_t *part = (uint64_t *)d_part;
+    // per-workgroup counter rows: setup's nblocks rows, then one raster row per tile
+    uint64_t *part = (uint64_t *)d_part, *part_r = part + 4ull * nblocks;
+    uint64_t *h_out = (uint64_t *)h_pinned;
     uint32_t cap = pair_cap;
-    // The device steps of the pass in three stages (kernels only, so each stage can be captured
-    // into a graph): 0 setup + scan, 1 binning, 2 raster. The timing events go between stages,
-    // outside any graph.
+    // The device steps of the pass in four stages, kernels only, captured together into one
+    // graph: 0 setup + count scan, 1 binning, 2 raster, 3 finish (counter sums, pair total and
+    // the stage stamps written into host memory). Setup, emit and raster stamp their own start.
     const int sort_passes = (BitsFor(tile_total) + 7) / 8;
     const bool in_alt = tri_total && (sort_passes & 1);
     uint32_t *bins = in_alt ? (uint32_t *)d_vals2 : (uint32_t *)d_vals;
@@ -893,34 +887,30 @@ VkResult Queue::RunPass(const std::vector<Op *> &passes, bool fused_clear, ExecS
     if((e = (x)) != hipSuccess) \
       return e;                 \
   } while(0)
-      uint64_t *stamps = (uint64_t *)d_stamps;
+      uint64_t *stamps = (uint64_t *)d_stamps, *stamp_r = stamps + 2;
       if(stage == 0)
       {
-        VA_TRY(va_stamp_async(stamps, stream));
         if(tri_total)
         {
-          // the setup kernel zeroes the counters and the scan's last count itself
+          // the setup kernel zeroes the scan's last count and writes stamp 0 itself
           uint2 *einfo = (uint2 *)d_einfo;
-          uint32_t *ctr_words = (uint32_t *)d_ctr;
-          void *args[] = {&dd, &bd, &df, (void *)&ts, &rec, &cnt, &part, &einfo, &tri_total, &ctr_words};
+          void *args[] = {&dd, &bd, &df, (void *)&ts, &rec, &cnt, &part, &einfo, &tri_total, &stamps};
           VA_TRY(hipModuleLaunchKernel(mod->setup, nblocks, 1, 1, 256, 1, 1, 0, stream, args, nullptr));
-          VA_TRY(va_reduce_cols_async(part, nblocks, 4, (uint64_t *)&ctr->tris_in, stream));
           VA_TRY(va_scan_u32((uint32_t *)d_temp, cnt, offs, tri_total + 1, stream));
         }
         else
         {
-          VA_TRY(va_fill_u32_async(d_ctr, sizeof(VaCounters) / 4, 0u, stream));
+          VA_TRY(va_stamp_async(stamps, stream));
           VA_TRY(va_fill_u32_async(offs, 1, 0u, stream));
         }
       }
       else if(stage == 1)
       {
-        VA_TRY(va_stamp_async(stamps + 1, stream));
         if(tri_total)
         {
           uint32_t *keys = (uint32_t *)d_keys, *vals = (uint32_t *)d_vals;
           uint2 *einfo = (uint2 *)d_einfo;
-          void *eargs[] = {&einfo, &offs, &tri_total, &cap, &keys, &vals};
+          void *eargs[] = {&einfo, &offs, &tri_total, &cap, &keys, &vals, &stamps};
           VA_TRY(hipModuleLaunchKernel(mod->emit, (tri_total + 255) / 256, 1, 1, 256, 1, 1, 0, stream, eargs, nullptr));
           int alt = 0;
           VA_TRY(va_sort_pairs((uint32_t *)d_stemp, keys, vals, (uint32_t *)d_keys2, (uint32_t *)d_vals2, total_dev, cap,
@@ -932,29 +922,33 @@ VkResult Queue::RunPass(const std::vector<Op *> &passes, bool fused_clear, ExecS
           VA_TRY(hipModuleLaunchKernel(mod->ranges, (tile_total + 256) / 256, 1, 1, 256, 1, 1, 0, stream, rargs, nullptr));
         }
         else
+        {
+          VA_TRY(va_stamp_async(stamps + 1, stream));
           VA_TRY(va_fill_u32_async(d_start, tile_total + 1, 0u, stream));
+        }
       }
+      else if(stage == 3)
+        VA_TRY(va_finish_async(part, tri_total ? nblocks : 0u, part_r, tile_total, total_dev, stamps, h_out, stream));
       else
       {
         // tile t's bin is [start[t], start[t + 1])
         uint32_t *st_ = (uint32_t *)d_start, *en = (uint32_t *)d_start + 1;
         uint32_t use_fused = fused_clear ? 1u : 0u;
-        void *args[] = {&dd, &df, (void *)&nframes, &rec, &bins, &st_, &en, &tile_total, &use_fused, &part, &total_dev, &cap};
+        void *args[] = {&dd,         &df,        (void *)&nframes, &rec,      &bins, &st_,    &en,
+                        &tile_total, &use_fused, &part_r,          &total_dev, &cap,  &stamp_r};
         uint32_t threads = (T / 8) * (T / 8) * 64;
         if(cfg.persistent)
         {
           uint32_t *tile_counter = (uint32_t *)d_stamps + 8;
           VA_TRY(va_fill_u32_async(tile_counter, 1, 0u, stream));
-          void *pargs[] = {&dd, &df,   (void *)&nframes, &rec,        &bins, &st_,         &en,
-                           &tile_total, &use_fused, &part, &total_dev, &cap, &tile_counter};
+          void *pargs[] = {&dd,         &df,        (void *)&nframes, &rec,       &bins, &st_,          &en,
+                           &tile_total, &use_fused, &part_r,          &total_dev, &cap,  &tile_counter, &stamp_r};
           const uint32_t grid = std::min(mod->raster_p_grid[ts - 3], std::max(tile_total, 1u));
           VA_TRY(hipModuleLaunchKernel(mod->raster_p[ts - 3], grid, 1, 1, threads, 1, 1, 0, stream, pargs, nullptr));
-          VA_TRY(va_reduce_cols_async(part, tile_total, 4, (uint64_t *)&ctr->pixels_written, stream));
           return hipSuccess;
         }
         hipFunction_t k = mod->raster[ts - 3];
         VA_TRY(hipModuleLaunchKernel(k, tile_total, 1, 1, threads, 1, 1, 0, stream, args, nullptr));
-        VA_TRY(va_reduce_cols_async(part, tile_total, 4, (uint64_t *)&ctr->pixels_written, stream));
       }
 #undef VA_TRY
       return hipSuccess;
@@ -976,7 +970,7 @@ VkResult Queue::RunPass(const std::vector<Op *> &passes, bool fused_clear, ExecS
       key.dims[6] = fused_clear ? 1u : 0u;
       key.dims[7] = cfg.persistent;
       void *bufs[] = {d_frames, d_draws, d_rec, d_count, d_offs, d_keys, d_vals, d_keys2, d_vals2,
-                      d_stemp, d_temp, d_start, d_einfo, d_ctr, d_blocks, d_part, d_stamps};
+                      d_stemp, d_temp, d_start, d_einfo, d_blocks, d_part, d_stamps, h_pinned};
       static_assert(sizeof(bufs) <= sizeof(key.bufs), "graph key holds every buffer");
       memcpy(key.bufs, bufs, sizeof(bufs));
       for(size_t i = 0; i < graph_cache.size(); i++)
@@ -991,19 +985,16 @@ VkResult Queue::RunPass(const std::vector<Op *> &passes, bool fused_clear, ExecS
         GraphSet set;
         set.key = key;
         hipError_t err = hipSuccess;
-        for(int part = 0; part < 2 && err == hipSuccess; part++)
         {
           hipError_t e = hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal);
           hipError_t er = e;
-          if(e == hipSuccess)
-            er = part == 0 ? enqueue(0) : enqueue(2);
-          if(e == hipSuccess && er == hipSuccess && part == 0)
-            er = enqueue(1);
+          for(int stage = 0; stage < 4 && er == hipSuccess; stage++)
+            er = enqueue(stage);
           hipGraph_t g = nullptr;
           hipError_t ec = e == hipSuccess ? hipStreamEndCapture(stream, &g) : e;
           err = er != hipSuccess ? er : ec;
           if(err == hipSuccess && g)
-            err = hipGraphInstantiate(&set.exec[part], g, nullptr, nullptr, 0);
+            err = hipGraphInstantiate(&set.exec[0], g, nullptr, nullptr, 0);
           else if(err == hipSuccess)
             err = hipErrorUnknown;
           if(g)
@@ -1031,29 +1022,17 @@ VkResult Queue::RunPass(const std::vector<Op *> &passes, bool fused_clear, ExecS
         }
       }
     }
-    // setup and binning as one launch (one graph), then the raster stage, with HIP events
-    // around both; the setup / binning split comes from the device stamps
-    HIPCHK(hipEventRecord(ev[0], stream));
+    // the whole pass is one graph launch; the stage times come from the device stamps
     if(gs)
-      HIPCHK(hipGraphLaunch(gs->exec[0], stream));
-    else
     {
-      HIPCHK(enqueue(0));
-      HIPCHK(enqueue(1));
-    }
-    HIPCHK(hipEventRecord(ev[2], stream));
-    if(gs)
-      HIPCHK(hipGraphLaunch(gs->exec[1], stream));
-    else
-      HIPCHK(enqueue(2));
-    HIPCHK(hipEventRecord(ev[3], stream));
-    if(gs)
+      HIPCHK(hipGraphLaunch(gs->exec[0], stream));
       st.graph_launches++;
-    HIPCHK(hipMemcpyAsync(h_pinned + 64, d_ctr, sizeof(VaCounters), hipMemcpyDeviceToHost, stream));
-    HIPCHK(hipMemcpyAsync(h_pinned + 128, d_stamps, 16, hipMemcpyDeviceToHost, stream));
-    HIPCHK(hipMemcpyAsync(h_pinned, total_dev, 4, hipMemcpyDeviceToHost, stream));
+    }
+    else
+      for(int stage = 0; stage < 4; stage++)
+        HIPCHK(enqueue(stage));
     HIPCHK(hipStreamSynchronize(stream));
-    total_pairs = h_pinned[0];
+    total_pairs = (uint32_t)h_out[0];
     if(total_pairs <= pair_cap)
       break;
     if(overflow_retries++ > 0)
@@ -1064,31 +1043,25 @@ VkResult Queue::RunPass(const std::vector<Op *> &passes, bool fused_clear, ExecS
     pair_cap = total_pairs + total_pairs / 4 + 4096;
   }
   VaCounters c;
-  memcpy(&c, h_pinned + 64, sizeof(c));
+  memset(&c, 0, sizeof(c));
+  const uint64_t *h_out = (const uint64_t *)h_pinned;
+  memcpy(&c.tris_in, h_out + 2, 8 * sizeof(uint64_t));  // tris_in .. shaded_dwrite
   c.draws = ndraws;
   st.last_cfg = cfg;
   {
-    float t02 = 0.0f, t23 = 0.0f;
-    hipError_t e1 = hipEventElapsedTime(&t02, ev[0], ev[2]);
-    hipError_t e2 = hipEventElapsedTime(&t23, ev[2], ev[3]);
-    if(e1 != hipSuccess || e2 != hipSuccess)
+    // stamps: 0 setup start, 1 binning start, 2 raster start, 3 finish start (constant-rate clock)
+    const uint64_t s0 = h_out[10], s1 = h_out[11], s2 = h_out[12], s3 = h_out[13];
+    const double khz = (double)Device::Get().wall_khz;
+    if(khz > 0.0 && s0 <= s1 && s1 <= s2 && s2 <= s3)
     {
-      (void)hipGetLastError();
-      LogError("stage timing unavailable: %s", hipGetErrorString(e1 != hipSuccess ? e1 : e2));
+      st.setup_ms += (double)(s1 - s0) / khz;
+      st.bin_ms += (double)(s2 - s1) / khz;
+      st.raster_ms += (double)(s3 - s2) / khz;
+      if(tuning)
+        RecordTuning(sig, cfg, (double)(s3 - s0) / khz);
     }
     else
-    {
-      uint64_t s0, s1;
-      memcpy(&s0, h_pinned + 128, 8);
-      memcpy(&s1, h_pinned + 130, 8);
-      double setup = s1 > s0 ? (double)(s1 - s0) / (double)Device::Get().wall_khz : 0.0;
-      setup = std::min(setup, (double)t02);
-      st.setup_ms += setup;
-      st.bin_ms += t02 - setup;
-      st.raster_ms += t23;
-      if(tuning)
-        RecordTuning(sig, cfg, (double)t02 + t23);
-    }
+      LogError("stage timing unavailable (device clock stamps out of order)");
   }
 
   st.raster_launches++;

@@ -106,21 +106,19 @@ VA_INLINE u32 va_index(const VaDraw &d, u32 i)
 // s_rec4[q * 257 + t], padded against bank conflicts) and written out as the block's contiguous
 // record range with coalesced 16-byte stores.
 // emit_info[g] = (first tile key, tiles per bbox row | frame tiles per row << 16) for va_emit.
-// Workgroup 0 also zeroes the frame counters and the count-scan's last element.
+// Workgroup 0 also zeroes the count-scan's last element and stamps the start of the pass.
 extern "C" __global__ __launch_bounds__(256) void va_setup(
     const VaDraw *__restrict__ draws, const u32 *__restrict__ block_draw, const VaFrame *__restrict__ frames,
     u32 tile_shift, u32 *__restrict__ rec, u32 *__restrict__ tile_count, u64 *__restrict__ part,
-    uint2 *__restrict__ emit_info, u32 total_tris, u32 *__restrict__ counters)
+    uint2 *__restrict__ emit_info, u32 total_tris, u64 *__restrict__ stamps)
 {
   constexpr u32 Q = VA_REC_STRIDE / 4u;
   __shared__ uint4 s_rec4[Q * 257];  // 16-byte group q of thread t's record at [q * 257 + t]
   __shared__ u32 s_keep[256];        // 1 for threads whose primitive produced a record
-  if(blockIdx.x == 0u)
+  if(blockIdx.x == 0u && threadIdx.x == 0u)
   {
-    if(threadIdx.x < (u32)(sizeof(VaCounters) / 4))
-      counters[threadIdx.x] = 0u;
-    if(threadIdx.x == 0u)
-      tile_count[total_tris] = 0u;
+    stamps[0] = (u64)wall_clock64();
+    tile_count[total_tris] = 0u;
   }
   const u32 di = __builtin_amdgcn_readfirstlane(block_draw[blockIdx.x]);
   const VaDraw &d = draws[di];
@@ -256,14 +254,16 @@ extern "C" __global__ __launch_bounds__(256) void va_setup(
 // the pair total); the group's offsets and emit descriptors are staged in LDS, and each thread
 // takes pairs of the group by stride, finding its primitive by a binary search over the 257
 // offsets. Pairs at or beyond cap are not written (the host sees the total and grows the
-// buffers).
+// buffers). Workgroup 0 stamps the start of binning.
 extern "C" __global__ __launch_bounds__(256) void va_emit(
     const uint2 *__restrict__ emit_info, const u32 *__restrict__ offsets, u32 total_tris, u32 cap,
-    u32 *__restrict__ keys, u32 *__restrict__ vals)
+    u32 *__restrict__ keys, u32 *__restrict__ vals, u64 *__restrict__ stamps)
 {
   __shared__ u32 s_off[257];
   __shared__ uint2 s_info[256];
   const u32 t0 = blockIdx.x * 256u;
+  if(blockIdx.x == 0u && threadIdx.x == 0u)
+    stamps[1] = (u64)wall_clock64();
   s_off[threadIdx.x] = offsets[min(t0 + threadIdx.x, total_tris)];
   if(threadIdx.x == 0u)
     s_off[256] = offsets[min(t0 + 256u, total_tris)];
@@ -808,15 +808,18 @@ VA_INLINE void va_raster_tile(const VaDraw *__restrict__ draws, const VaFrame *_
 }
 
 // pair_total / pair_cap: when the pairs overflowed the bins the launch writes nothing, so the
-// host can grow the buffers and run the pass again on an untouched target.
+// host can grow the buffers and run the pass again on an untouched target. Workgroup 0 stamps
+// the start of the raster stage.
 #define VA_RASTER_KERNEL(T)                                                                                 \
   extern "C" __global__ __launch_bounds__((T / 8) * (T / 8) * 64) void va_raster_##T(                      \
       const VaDraw *__restrict__ draws, const VaFrame *__restrict__ frames, u32 nframes,                  \
       const u32 *__restrict__ rec, const u32 *__restrict__ bins, const u32 *__restrict__ bin_start,        \
       const u32 *__restrict__ bin_end, u32 ntiles, u32 use_fused_clear, u64 *__restrict__ part,            \
-      const u32 *__restrict__ pair_total, u32 pair_cap)                                                     \
+      const u32 *__restrict__ pair_total, u32 pair_cap, u64 *__restrict__ stamp)                            \
   {                                                                                                         \
     const u32 tile = blockIdx.x;                                                                            \
+    if(tile == 0u && threadIdx.x == 0u)                                                                     \
+      *stamp = (u64)wall_clock64();                                                                         \
     if(tile < ntiles && *pair_total <= pair_cap)                                                            \
       va_raster_tile<T>(draws, frames, nframes, rec, bins, bin_start, bin_end, use_fused_clear, part, tile); \
   }
@@ -832,9 +835,11 @@ VA_RASTER_KERNEL(32)
       const VaDraw *__restrict__ draws, const VaFrame *__restrict__ frames, u32 nframes,                    \
       const u32 *__restrict__ rec, const u32 *__restrict__ bins, const u32 *__restrict__ bin_start,          \
       const u32 *__restrict__ bin_end, u32 ntiles, u32 use_fused_clear, u64 *__restrict__ part,              \
-      const u32 *__restrict__ pair_total, u32 pair_cap, u32 *__restrict__ tile_counter)                       \
+      const u32 *__restrict__ pair_total, u32 pair_cap, u32 *__restrict__ tile_counter, u64 *__restrict__ stamp) \
   {                                                                                                           \
     __shared__ u32 s_next;                                                                                    \
+    if(blockIdx.x == 0u && threadIdx.x == 0u)                                                                 \
+      *stamp = (u64)wall_clock64();                                                                           \
     if(*pair_total > pair_cap)                                                                                \
       return;                                                                                                 \
     for(;;)                                                                                                   \
