@@ -103,6 +103,10 @@ Queue::~Queue()
     for(hipGraphExec_t x : g.exec)
       if(x)
         hipGraphExecDestroy(x);
+  if(ev_begin)
+    hipEventDestroy(ev_begin);
+  if(ev_end)
+    hipEventDestroy(ev_end);
   if(stream)
     hipStreamDestroy(stream);
 }
@@ -275,6 +279,8 @@ static void AddMem(std::vector<VkDeviceMemory_T *> &v, VkDeviceMemory_T *m)
     v.push_back(m);
 }
 
+std::atomic<uint64_t> g_descriptor_writes{0};
+
 bool Queue::Decode(const VkCommandBuffer_T *cb, Submission &sub, std::string &err)
 {
   const uint8_t *cur = cb->stream.data(), *end = cur + cb->stream.size();
@@ -289,6 +295,10 @@ bool Queue::Decode(const VkCommandBuffer_T *cb, Submission &sub, std::string &er
   VkDeviceSize vb_off[4] = {};
   int pass = -1;
   std::set<const VkDeviceMemory_T *> touched;  // memories written earlier in this command buffer
+  sub.cb = cb;
+  sub.recording = cb->recording;
+  const uint64_t desc_writes = g_descriptor_writes.load();
+  sub.desc_writes = desc_writes;
   auto pull = [&](size_t n) {
     const uint8_t *p = cur;
     cur += n;
@@ -366,6 +376,8 @@ bool Queue::Decode(const VkCommandBuffer_T *cb, Submission &sub, std::string &er
           AddMem(sub.downloads, op.frame.depth->mem);
           touched.insert(op.frame.depth->mem);
         }
+        op.key = cb->recording && sub.ops.size() < 65536u ? (cb->recording << 16 | sub.ops.size()) : 0u;
+        op.desc_writes = desc_writes;
         sub.ops.push_back(std::move(op));
         pass = (int)sub.ops.size() - 1;
         break;
@@ -716,59 +728,88 @@ void Queue::RecordTuning(uint64_t sig, const TileConfig &cfg, double ms)
 
 VkResult Queue::RunPass(const std::vector<Op *> &passes, bool fused_clear, ExecStats &st)
 {
+  // a single pass decoded from a recording reuses the tables built for it last time
+  const bool reusable = passes.size() == 1 && passes[0]->key != 0;
+  const bool have = reusable && tables_cache.key == passes[0]->key &&
+                    tables_cache.desc_writes == passes[0]->desc_writes;
+  PassTables local;
+  PassTables &tb = reusable ? tables_cache : local;
+  if(!have)
+  {
+    tb.key = 0;  // invalid until rebuilt below
+    tb.sig = PassSignature(passes);
+  }
   bool tuning = false;
-  const uint64_t sig = PassSignature(passes);
+  const uint64_t sig = tb.sig;
   const TileConfig cfg = ChooseConfig(sig, &tuning);
   const uint32_t ts = cfg.tile_shift, T = 1u << ts;
-  std::vector<VaFrame> frames;
-  std::vector<VaDraw> draws;
-  std::vector<std::shared_ptr<ShaderPair>> pairs;
-  std::vector<uint32_t> block_draw;  // setup workgroup -> draw
-  uint32_t tile_total = 0, tri_total = 0;
-  for(Op *p : passes)
+  std::vector<VaFrame> &frames = tb.frames;
+  std::vector<VaDraw> &draws = tb.draws;
+  std::vector<std::shared_ptr<ShaderPair>> &pairs = tb.pairs;
+  std::vector<uint32_t> &block_draw = tb.block_draw;  // setup workgroup -> draw
+  uint32_t &tile_total = tb.tile_total, &tri_total = tb.tri_total;
+  if(!(have && tb.tile_shift == ts && tb.fused_clear == fused_clear))
   {
-    VaFrame fr;
-    memset(&fr, 0, sizeof(fr));
-    VkImage_T *c = p->frame.color;
-    fr.color = (unsigned long long)(uintptr_t)c->devPixels();
-    fr.depth = p->frame.depth ? (unsigned long long)(uintptr_t)p->frame.depth->devPixels() : 0ull;
-    fr.width = c->extent.width;
-    fr.height = c->extent.height;
-    fr.color_bpp = c->bytesPerPixel;
-    if(fused_clear)
+    frames.clear();
+    draws.clear();
+    pairs.clear();
+    block_draw.clear();
+    tile_total = 0;
+    tri_total = 0;
+    for(Op *p : passes)
     {
-      fr.clear_flags = (p->frame.clear_color ? VA_CLEAR_COLOR : 0u) | (p->frame.clear_depth && p->frame.depth ? VA_CLEAR_DEPTH : 0u);
-      memcpy(fr.clear_color, p->frame.clear_col.color.float32, 16);
-      fr.clear_depth = p->frame.clear_dep.depthStencil.depth;
-    }
-    fr.tiles_x = (fr.width + T - 1) >> ts;
-    fr.tiles_y = (fr.height + T - 1) >> ts;
-    fr.tile_base = tile_total;
-    if(fr.width >= 65535 || fr.height >= 65535)
-    {
-      LogError("render target larger than 65534 pixels per side");
-      return VK_ERROR_FEATURE_NOT_PRESENT;
-    }
-    tile_total += fr.tiles_x * fr.tiles_y;
-    for(DrawDesc &dd : p->draws)
-    {
-      VaDraw v = dd.d;
-      v.frame = (uint32_t)frames.size();
-      auto it = std::find(pairs.begin(), pairs.end(), dd.pipeline->pair);
-      if(it == pairs.end())
+      VaFrame fr;
+      memset(&fr, 0, sizeof(fr));
+      VkImage_T *c = p->frame.color;
+      fr.color = (unsigned long long)(uintptr_t)c->devPixels();
+      fr.depth = p->frame.depth ? (unsigned long long)(uintptr_t)p->frame.depth->devPixels() : 0ull;
+      fr.width = c->extent.width;
+      fr.height = c->extent.height;
+      fr.color_bpp = c->bytesPerPixel;
+      if(fused_clear)
       {
-        pairs.push_back(dd.pipeline->pair);
-        it = pairs.end() - 1;
+        fr.clear_flags = (p->frame.clear_color ? VA_CLEAR_COLOR : 0u) | (p->frame.clear_depth && p->frame.depth ? VA_CLEAR_DEPTH : 0u);
+        memcpy(fr.clear_color, p->frame.clear_col.color.float32, 16);
+        fr.clear_depth = p->frame.clear_dep.depthStencil.depth;
       }
-      v.slot = (uint32_t)(it - pairs.begin());
-      v.tri_base = tri_total;
-      tri_total += v.ntris;
-      v.block_base = (uint32_t)block_draw.size();
-      for(uint32_t b = 0; b < (v.ntris + 255) / 256; b++)
-        block_draw.push_back((uint32_t)draws.size());
-      draws.push_back(v);
+      fr.tiles_x = (fr.width + T - 1) >> ts;
+      fr.tiles_y = (fr.height + T - 1) >> ts;
+      fr.tile_base = tile_total;
+      if(fr.width >= 65535 || fr.height >= 65535)
+      {
+        LogError("render target larger than 65534 pixels per side");
+        return VK_ERROR_FEATURE_NOT_PRESENT;
+      }
+      tile_total += fr.tiles_x * fr.tiles_y;
+      for(DrawDesc &dd : p->draws)
+      {
+        VaDraw v = dd.d;
+        v.frame = (uint32_t)frames.size();
+        auto it = std::find(pairs.begin(), pairs.end(), dd.pipeline->pair);
+        if(it == pairs.end())
+        {
+          pairs.push_back(dd.pipeline->pair);
+          it = pairs.end() - 1;
+        }
+        v.slot = (uint32_t)(it - pairs.begin());
+        v.tri_base = tri_total;
+        tri_total += v.ntris;
+        v.block_base = (uint32_t)block_draw.size();
+        for(uint32_t b = 0; b < (v.ntris + 255) / 256; b++)
+          block_draw.push_back((uint32_t)draws.size());
+        draws.push_back(v);
+      }
+      frames.push_back(fr);
     }
-    frames.push_back(fr);
+    tb.version = 0;
+    if(reusable)
+    {
+      tb.key = passes[0]->key;
+      tb.desc_writes = passes[0]->desc_writes;
+      tb.tile_shift = ts;
+      tb.fused_clear = fused_clear;
+      tb.version = ++table_versions;
+    }
   }
   bool any_clear = false;
   for(auto &f : frames)
@@ -829,7 +870,12 @@ VkResult Queue::RunPass(const std::vector<Op *> &passes, bool fused_clear, ExecS
       HIPCHK(TranscodeAsync(im, stream));
   }
 
-  // the per-pass tables go up only when they changed (a re-submitted command buffer reuses them)
+  // the per-pass tables go up only when they changed (a re-submitted command buffer reuses them;
+  // tables reused from the cache and already resident are not even compared)
+  const bool resident = tb.version != 0 && up_version == tb.version && d_frames == up_frames_ptr &&
+                        d_draws == up_draws_ptr && d_blocks == up_blocks_ptr;
+  if(!resident)
+  {
   if(d_frames != up_frames_ptr || frames.size() != up_frames.size() ||
      memcmp(frames.data(), up_frames.data(), sizeof(VaFrame) * nframes) != 0)
   {
@@ -849,6 +895,8 @@ VkResult Queue::RunPass(const std::vector<Op *> &passes, bool fused_clear, ExecS
     HIPCHK(hipMemcpyAsync(d_blocks, block_draw.data(), 4 * nblocks, hipMemcpyHostToDevice, stream));
     up_blocks = block_draw;
     up_blocks_ptr = d_blocks;
+  }
+  up_version = tb.version;
   }
 
   // The pair buffers are sized by a capacity that only grows: the device work never waits for
@@ -1031,6 +1079,9 @@ VkResult Queue::RunPass(const std::vector<Op *> &passes, bool fused_clear, ExecS
     else
       for(int stage = 0; stage < 4; stage++)
         HIPCHK(enqueue(stage));
+    if(!ev_end)
+      HIPCHK(hipEventCreate(&ev_end));
+    HIPCHK(hipEventRecord(ev_end, stream));
     HIPCHK(hipStreamSynchronize(stream));
     total_pairs = (uint32_t)h_out[0];
     if(total_pairs <= pair_cap)
@@ -1064,6 +1115,7 @@ VkResult Queue::RunPass(const std::vector<Op *> &passes, bool fused_clear, ExecS
       LogError("stage timing unavailable (device clock stamps out of order)");
   }
 
+  drained = true;  // everything enqueued so far has completed, ev_end marks its end
   st.raster_launches++;
   st.total_tris += tri_total;
   st.total_pairs += total_pairs;
@@ -1096,19 +1148,22 @@ VkResult Queue::Execute(std::vector<Submission> &subs)
   if(!stream)
     HIPCHK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
   VkResult res = VK_SUCCESS;
+  if(!ev_begin)
+    HIPCHK(hipEventCreate(&ev_begin));
+  if(!ev_end)
+    HIPCHK(hipEventCreate(&ev_end));
   for(Submission &sub : subs)
   {
     ExecStats st;
-    hipEvent_t e0 = nullptr, e1 = nullptr;
-    HIPCHK(hipEventCreate(&e0));
-    HIPCHK(hipEventCreate(&e1));
     for(VkDeviceMemory_T *m : sub.uploads)
       HIPCHK(hipMemcpyAsync(m->dev, m->host, m->size, hipMemcpyHostToDevice, stream));
-    HIPCHK(hipEventRecord(e0, stream));
+    HIPCHK(hipEventRecord(ev_begin, stream));
+    drained = false;
     for(Op &op : sub.ops)
     {
       if(res != VK_SUCCESS)
         break;
+      drained = false;  // RunPass sets it again once its work has completed
       if(op.kind == Op::Copy)
         res = RunCopy(op.copy);
       else if(mode == ExecMode::Batched)
@@ -1126,21 +1181,29 @@ VkResult Queue::Execute(std::vector<Submission> &subs)
         }
       }
     }
-    HIPCHK(hipEventRecord(e1, stream));
+    // one synchronization per submission: none more when the last operation was a pass that
+    // has already waited for its work and nothing needs copying back
+    bool need_sync = !drained;
+    if(!drained)
+      HIPCHK(hipEventRecord(ev_end, stream));
     for(VkDeviceMemory_T *m : sub.downloads)
+    {
       HIPCHK(hipMemcpyAsync(m->host, m->dev, m->size, hipMemcpyDeviceToHost, stream));
-    hipError_t e = hipStreamSynchronize(stream);
-    if(e != hipSuccess && res == VK_SUCCESS)
-      res = HipToVk(e);
+      need_sync = true;
+    }
+    if(need_sync)
+    {
+      hipError_t e = hipStreamSynchronize(stream);
+      if(e != hipSuccess && res == VK_SUCCESS)
+        res = HipToVk(e);
+    }
     float ms = 0.0f;
-    if(hipEventElapsedTime(&ms, e0, e1) != hipSuccess)
+    if(hipEventElapsedTime(&ms, ev_begin, ev_end) != hipSuccess)
     {
       ms = 0.0f;
       (void)hipGetLastError();
     }
     st.gpu_ms_total = ms;
-    hipEventDestroy(e0);
-    hipEventDestroy(e1);
     last_stats = st;
     accum_stats.gpu_ms_total += st.gpu_ms_total;
     accum_stats.raster_ms += st.raster_ms;
@@ -1196,12 +1259,28 @@ VkResult Queue::Submit(uint32_t count, const VkSubmitInfo *infos, VkFence fence)
   if(!Device::Get().ok())
     return VK_ERROR_DEVICE_LOST;
   std::vector<Submission> subs;
+  // a synchronous submit of one command buffer keeps its decoded form for the next submit of
+  // the same recording (no decode, and RunPass reuses the pass tables built from it)
+  const bool single = !async && count == 1 && infos[0].commandBufferCount == 1;
   for(uint32_t i = 0; i < count; i++)
     for(uint32_t c = 0; c < infos[i].commandBufferCount; c++)
     {
       Submission s;
       std::string err;
-      if(!Decode(infos[i].pCommandBuffers[c], s, err))
+      const VkCommandBuffer_T *cbp = infos[i].pCommandBuffers[c];
+      bool reused = false;
+      if(single && cbp->recording != 0)
+      {
+        std::lock_guard<std::mutex> lk(decoded_m);
+        if(decoded_valid && decoded.cb == cbp && decoded.recording == cbp->recording &&
+           decoded.desc_writes == g_descriptor_writes.load())
+        {
+          s = std::move(decoded);
+          decoded_valid = false;
+          reused = true;
+        }
+      }
+      if(!reused && !Decode(cbp, s, err))
       {
         LogError("vkQueueSubmit: %s", err.c_str());
         return VK_ERROR_FEATURE_NOT_PRESENT;
@@ -1221,7 +1300,18 @@ VkResult Queue::Submit(uint32_t count, const VkSubmitInfo *infos, VkFence fence)
   }
   subs.back().fence = fence;
   if(!async)
-    return Execute(subs);  // synchronous, as the reference's vkQueueSubmit (cmd_exec.cpp:187-201)
+  {
+    // synchronous, as the reference's vkQueueSubmit (cmd_exec.cpp:187-201)
+    VkResult r = Execute(subs);
+    if(single && r == VK_SUCCESS && subs[0].recording != 0)
+    {
+      std::lock_guard<std::mutex> lk(decoded_m);
+      decoded = std::move(subs[0]);
+      decoded.fence = nullptr;
+      decoded_valid = true;
+    }
+    return r;
+  }
   {
     std::lock_guard<std::mutex> lk(qm);
     if(!worker.joinable())

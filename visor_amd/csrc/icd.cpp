@@ -6,6 +6,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cstdio>
 #include <cstdlib>
@@ -590,6 +591,7 @@ VKAPI_ATTR VkResult VKAPI_CALL vkFreeDescriptorSets(VkDevice, VkDescriptorPool, 
 VKAPI_ATTR void VKAPI_CALL vkUpdateDescriptorSets(VkDevice, uint32_t nw, const VkWriteDescriptorSet *writes, uint32_t nc,
                                                   const VkCopyDescriptorSet *copies)
 {
+  visor::g_descriptor_writes.fetch_add(1);
   for(uint32_t i = 0; i < nw; i++)
   {
     const VkWriteDescriptorSet &w = writes[i];
@@ -836,7 +838,9 @@ VKAPI_ATTR void VKAPI_CALL vkFreeCommandBuffers(VkDevice, VkCommandPool, uint32_
 
 VKAPI_ATTR VkResult VKAPI_CALL vkBeginCommandBuffer(VkCommandBuffer cb, const VkCommandBufferBeginInfo *)
 {
+  static std::atomic<uint64_t> next_recording{1};
   cb->stream.clear();
+  cb->recording = next_recording.fetch_add(1);
   return VK_SUCCESS;
 }
 

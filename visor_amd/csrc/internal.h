@@ -199,6 +199,7 @@ struct VkCommandBuffer_T
 {
   uintptr_t loaderMagic;
   bool live = false;
+  uint64_t recording = 0;  // unique per vkBeginCommandBuffer (never reused)
   std::vector<uint8_t> stream;
 
   template <typename T>
@@ -309,10 +310,20 @@ struct Op
   FrameDesc frame;
   std::vector<DrawDesc> draws;
   CopyDesc copy;
+  // (recording << 16 | index) for a pass decoded from a command buffer recording, so the
+  // tables built from it can be reused when the same recording is submitted again; 0 = none.
+  // desc_writes: the descriptor-write count at decode time (descriptors feed the tables).
+  uint64_t key = 0, desc_writes = 0;
 };
+
+// Count of vkUpdateDescriptorSets calls: decoded submissions and pass tables cached from a
+// recording are reused only while it is unchanged.
+extern std::atomic<uint64_t> g_descriptor_writes;
 
 struct Submission
 {
+  const VkCommandBuffer_T *cb = nullptr;  // the command buffer and recording decoded into ops
+  uint64_t recording = 0, desc_writes = 0;
   std::vector<Op> ops;
   std::vector<VkDeviceMemory_T *> uploads;    // host-visible memory read by the GPU
   std::vector<VkDeviceMemory_T *> downloads;  // host-visible memory written by the GPU
@@ -419,6 +430,10 @@ private:
 
   hipStream_t stream = nullptr;
   std::mutex submit_m;  // one submission executes at a time
+  // device time of a submission: ev_begin before its first operation, ev_end after its last;
+  // RunPass records ev_end before the synchronization it needs anyway and sets drained
+  hipEvent_t ev_begin = nullptr, ev_end = nullptr;
+  bool drained = false;
   // growable device scratch
   void *d_frames = nullptr, *d_draws = nullptr, *d_rec = nullptr, *d_count = nullptr, *d_offs = nullptr;
   void *d_keys = nullptr, *d_vals = nullptr, *d_keys2 = nullptr, *d_vals2 = nullptr;
@@ -438,6 +453,28 @@ private:
   std::vector<VaDraw> up_draws;
   std::vector<uint32_t> up_blocks;
   void *up_frames_ptr = nullptr, *up_draws_ptr = nullptr, *up_blocks_ptr = nullptr;
+  uint64_t up_version = 0;      // PassTables::version the device buffers hold (0 = compare)
+  uint64_t table_versions = 0;  // versions handed out to rebuilt reusable tables
+  // device tables of one pass, built on the host
+  struct PassTables
+  {
+    uint64_t key = 0, desc_writes = 0;  // Op::key / desc_writes they were built from
+    uint64_t version = 0;               // changes whenever the tables are rebuilt
+    uint32_t tile_shift = 0;
+    bool fused_clear = false;
+    uint64_t sig = 0;
+    std::vector<VaFrame> frames;
+    std::vector<VaDraw> draws;
+    std::vector<std::shared_ptr<ShaderPair>> pairs;
+    std::vector<uint32_t> block_draw;  // setup workgroup -> draw
+    uint32_t tile_total = 0, tri_total = 0;
+  };
+  PassTables tables_cache;  // the last reusable pass's tables
+  // the last single-command-buffer submission, decoded; reused when the same recording is
+  // submitted again (synchronous submits)
+  std::mutex decoded_m;
+  Submission decoded;
+  bool decoded_valid = false;
   // the captured device sequence of the last batched pass, replayed while its key matches
   struct GraphKey
   {
