@@ -850,11 +850,18 @@ VkResult Queue::RunPass(const std::vector<Op *> &passes, bool fused_clear, ExecS
   if(!Grow(&d_frames, c_frames, sizeof(VaFrame) * nframes) || !Grow(&d_draws, c_draws, sizeof(VaDraw) * std::max(ndraws, 1u)) ||
      !Grow(&d_rec, c_rec, (size_t)stride * 4 * std::max(tri_total, 1u)) || !Grow(&d_count, c_count, 4ull * (tri_total + 1)) ||
      !Grow(&d_offs, c_offs, 4ull * (tri_total + 1)) || !Grow(&d_start, c_start, 4ull * (tile_total + 1)) ||
-     !Grow(&d_einfo, c_einfo, 8ull * std::max(tri_total, 1u)) || !Grow(&d_stamps, c_stamps, 64) ||
+     !Grow(&d_einfo, c_einfo, 8ull * std::max(tri_total, 1u)) || !Grow(&d_stamps, c_stamps, 256) ||
      !Grow(&d_blocks, c_blocks, 4ull * std::max<size_t>(nblocks, 1)) ||
      !Grow(&d_part, c_part, 32ull * ((size_t)nblocks + tile_total + 1)) ||
      !Grow(&d_temp, c_temp, 4 * va_scan_temp_words(tri_total + 1)))
     return VK_ERROR_OUT_OF_DEVICE_MEMORY;
+  // d_stamps: stage stamps (u64 0-3), persistent tile counter (u32 8), finish accumulators (u64
+  // 8-15) and ticket (u32 32); the accumulators and the ticket start at zero
+  if(d_stamps != stamps_zeroed)
+  {
+    HIPCHK(va_fill_u32_async(d_stamps, 64, 0u, stream));
+    stamps_zeroed = d_stamps;
+  }
   if(!h_pinned)  // written by the finish kernel, read by the host after the stream completes
     HIPCHK(hipHostMalloc((void **)&h_pinned, 4096, hipHostMallocCoherent));
 
@@ -976,7 +983,8 @@ VkResult Queue::RunPass(const std::vector<Op *> &passes, bool fused_clear, ExecS
         }
       }
       else if(stage == 3)
-        VA_TRY(va_finish_async(part, tri_total ? nblocks : 0u, part_r, tile_total, total_dev, stamps, h_out, stream));
+        VA_TRY(va_finish_async(part, tri_total ? nblocks : 0u, part_r, tile_total, total_dev, stamps, stamps + 8,
+                               (uint32_t *)d_stamps + 32, h_out, stream));
       else
       {
         // tile t's bin is [start[t], start[t + 1])

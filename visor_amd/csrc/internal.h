@@ -454,6 +454,7 @@ private:
   std::vector<uint32_t> up_blocks;
   void *up_frames_ptr = nullptr, *up_draws_ptr = nullptr, *up_blocks_ptr = nullptr;
   uint64_t up_version = 0;      // PassTables::version the device buffers hold (0 = compare)
+  void *stamps_zeroed = nullptr;  // d_stamps allocation whose accumulators were zeroed
   uint64_t table_versions = 0;  // versions handed out to rebuilt reusable tables
   // device tables of one pass, built on the host
   struct PassTables

@@ -366,52 +366,69 @@ extern "C" hipError_t va_transcode_async(const uint8_t *src, uint8_t *dst, const
   return hipGetLastError();
 }
 
-// End of a pass, one workgroup: sums the setup rows and the raster rows (4 counters each, in
-// VaCounters order from tris_in) and writes them, the pair total and the stage stamps straight
-// into host-visible memory, so the host reads one block after the stream completes.
-// out: [0] pair total, [2..9] counter sums, [10..12] stamps 0-2, [13] this kernel's start.
-__global__ __launch_bounds__(1024) void va_finish_kernel(const uint64_t *__restrict__ setup_part, uint32_t setup_rows,
-                                                         const uint64_t *__restrict__ raster_part, uint32_t raster_rows,
-                                                         const uint32_t *__restrict__ pair_total,
-                                                         const uint64_t *__restrict__ stamps, uint64_t *out)
+// End of a pass: sums the setup rows and the raster rows (4 counters each, in VaCounters order
+// from tris_in) and writes them, the pair total and the stage stamps straight into host-visible
+// memory, so the host reads one block after the stream completes.
+// out: [0] pair total, [2..9] counter sums, [10..12] stamps 0-2, [13] this launch's start.
+// Each workgroup adds its sums into acc[0..7] and takes a ticket; the workgroup that takes the
+// last ticket publishes the totals and returns acc and the ticket to zero for the next launch
+// (they start at zero when the buffer is allocated). No workgroup waits for another.
+constexpr uint32_t kFinishBlocks = 64;
+
+__global__ __launch_bounds__(256) void va_finish_kernel(const uint64_t *__restrict__ setup_part, uint32_t setup_rows,
+                                                        const uint64_t *__restrict__ raster_part, uint32_t raster_rows,
+                                                        const uint32_t *__restrict__ pair_total,
+                                                        const uint64_t *__restrict__ stamps, uint64_t *acc,
+                                                        uint32_t *ticket, uint64_t *out)
 {
-  __shared__ uint64_t s_sum[16][8];
+  __shared__ uint64_t s_sum[4][8];
+  __shared__ uint32_t s_last;
   const uint64_t t_start = (uint64_t)wall_clock64();
-  uint64_t acc[8];
+  uint64_t v[8];
 #pragma unroll
   for(int c = 0; c < 8; c++)
-    acc[c] = 0ull;
-  for(uint32_t r = threadIdx.x; r < setup_rows; r += 1024u)
+    v[c] = 0ull;
+  const uint32_t stride = gridDim.x * 256u;
+  for(uint32_t r = blockIdx.x * 256u + threadIdx.x; r < setup_rows; r += stride)
   {
 #pragma unroll
     for(int c = 0; c < 4; c++)
-      acc[c] += setup_part[(uint64_t)r * 4u + c];
+      v[c] += setup_part[(uint64_t)r * 4u + c];
   }
-  for(uint32_t r = threadIdx.x; r < raster_rows; r += 1024u)
+  for(uint32_t r = blockIdx.x * 256u + threadIdx.x; r < raster_rows; r += stride)
   {
 #pragma unroll
     for(int c = 0; c < 4; c++)
-      acc[4 + c] += raster_part[(uint64_t)r * 4u + c];
+      v[4 + c] += raster_part[(uint64_t)r * 4u + c];
   }
 #pragma unroll
   for(int c = 0; c < 8; c++)
     for(int o = 32; o > 0; o >>= 1)
-      acc[c] += (uint64_t)__shfl_xor((long long)acc[c], o);
+      v[c] += (uint64_t)__shfl_xor((long long)v[c], o);
   const uint32_t wave = threadIdx.x >> 6;
   if((threadIdx.x & 63u) == 0u)
   {
 #pragma unroll
     for(int c = 0; c < 8; c++)
-      s_sum[wave][c] = acc[c];
+      s_sum[wave][c] = v[c];
   }
   __syncthreads();
   if(threadIdx.x < 8u)
   {
-    uint64_t t = 0;
-    for(uint32_t w = 0; w < 16u; w++)
-      t += s_sum[w][threadIdx.x];
-    out[2 + threadIdx.x] = t;
+    const uint64_t t = s_sum[0][threadIdx.x] + s_sum[1][threadIdx.x] + s_sum[2][threadIdx.x] + s_sum[3][threadIdx.x];
+    if(t)
+      atomicAdd((unsigned long long *)&acc[threadIdx.x], (unsigned long long)t);
   }
+  __threadfence();
+  __syncthreads();
+  if(threadIdx.x == 0u)
+    s_last = atomicAdd(ticket, 1u) == gridDim.x - 1u ? 1u : 0u;
+  __syncthreads();
+  if(!s_last)
+    return;
+  __threadfence();
+  if(threadIdx.x < 8u)
+    out[2 + threadIdx.x] = (uint64_t)atomicExch((unsigned long long *)&acc[threadIdx.x], 0ull);
   if(threadIdx.x == 0u)
   {
     out[0] = *pair_total;
@@ -419,16 +436,17 @@ __global__ __launch_bounds__(1024) void va_finish_kernel(const uint64_t *__restr
     out[11] = stamps[1];
     out[12] = stamps[2];
     out[13] = t_start;
+    atomicExch(ticket, 0u);
   }
 }
 
 extern "C" hipError_t va_finish_async(const uint64_t *setup_part, uint32_t setup_rows, const uint64_t *raster_part,
                                       uint32_t raster_rows, const uint32_t *pair_total, const uint64_t *stamps,
-                                      uint64_t *out, hipStream_t stream)
+                                      uint64_t *acc, uint32_t *ticket, uint64_t *out, hipStream_t stream)
 {
   (void)hipGetLastError();
-  hipLaunchKernelGGL(va_finish_kernel, dim3(1), dim3(1024), 0, stream, setup_part, setup_rows, raster_part, raster_rows,
-                     pair_total, stamps, out);
+  hipLaunchKernelGGL(va_finish_kernel, dim3(kFinishBlocks), dim3(256), 0, stream, setup_part, setup_rows, raster_part,
+                     raster_rows, pair_total, stamps, acc, ticket, out);
   return hipGetLastError();
 }
 
