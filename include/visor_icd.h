@@ -44,7 +44,7 @@ VkResult vkQueueSubmit(VkQueue queue, uint32_t submitCount, const VkSubmitInfo *
 
 typedef struct VhStats {
   double gpu_ms;            /* device time of the submission's work (HIP events)          */
-  double raster_ms;         /* device time of the raster kernel(s)                        */
+  double raster_ms;         /* device time of the raster kernel(s) (device clock stamps)  */
   double setup_ms;          /* vertex program + triangle setup + count scan               */
   double bin_ms;            /* pair emit + stable sort + bin ranges                        */
   uint64_t raster_launches;
