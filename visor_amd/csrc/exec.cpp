@@ -850,7 +850,7 @@ VkResult Queue::RunPass(const std::vector<Op *> &passes, bool fused_clear, ExecS
   if(!Grow(&d_frames, c_frames, sizeof(VaFrame) * nframes) || !Grow(&d_draws, c_draws, sizeof(VaDraw) * std::max(ndraws, 1u)) ||
      !Grow(&d_rec, c_rec, (size_t)stride * 4 * std::max(tri_total, 1u)) || !Grow(&d_count, c_count, 4ull * (tri_total + 1)) ||
      !Grow(&d_offs, c_offs, 4ull * (tri_total + 1)) || !Grow(&d_start, c_start, 4ull * (tile_total + 1)) ||
-     !Grow(&d_einfo, c_einfo, 8ull * std::max(tri_total, 1u)) || !Grow(&d_stamps, c_stamps, 256) ||
+     !Grow(&d_einfo, c_einfo, 16ull * std::max(tri_total, 1u)) || !Grow(&d_stamps, c_stamps, 256) ||
      !Grow(&d_blocks, c_blocks, 4ull * std::max<size_t>(nblocks, 1)) ||
      !Grow(&d_part, c_part, 32ull * ((size_t)nblocks + tile_total + 1)) ||
      !Grow(&d_temp, c_temp, 4 * va_scan_temp_words(tri_total + 1)))
@@ -948,7 +948,7 @@ VkResult Queue::RunPass(const std::vector<Op *> &passes, bool fused_clear, ExecS
         if(tri_total)
         {
           // the setup kernel zeroes the scan's last count and writes stamp 0 itself
-          uint2 *einfo = (uint2 *)d_einfo;
+          uint4 *einfo = (uint4 *)d_einfo;
           void *args[] = {&dd, &bd, &df, (void *)&ts, &rec, &cnt, &part, &einfo, &tri_total, &stamps};
           VA_TRY(hipModuleLaunchKernel(mod->setup, nblocks, 1, 1, 256, 1, 1, 0, stream, args, nullptr));
           VA_TRY(va_scan_u32((uint32_t *)d_temp, cnt, offs, tri_total + 1, stream));
@@ -964,7 +964,7 @@ VkResult Queue::RunPass(const std::vector<Op *> &passes, bool fused_clear, ExecS
         if(tri_total)
         {
           uint32_t *keys = (uint32_t *)d_keys, *vals = (uint32_t *)d_vals;
-          uint2 *einfo = (uint2 *)d_einfo;
+          uint4 *einfo = (uint4 *)d_einfo;
           void *eargs[] = {&einfo, &offs, &tri_total, &cap, &keys, &vals, &stamps};
           VA_TRY(hipModuleLaunchKernel(mod->emit, (tri_total + 255) / 256, 1, 1, 256, 1, 1, 0, stream, eargs, nullptr));
           int alt = 0;

@@ -100,17 +100,70 @@ VA_INLINE u32 va_index(const VaDraw &d, u32 i)
   return 2ull * i + 2ull <= d.ib_bytes ? ((const u16 *)d.ib)[i] : 0u;
 }
 
+// True when the raster's 32-bit edge arithmetic cannot wrap for this triangle: window
+// coordinates within +-8191 and a target of at most 8192 pixels per side bound every product and
+// sum of the edge functions below 2^31, so exact integer tests equal the raster's.
+VA_INLINE bool va_small_coords(const i32 X[3], const i32 Y[3], const VaFrame &fr)
+{
+  bool ok = fr.width <= 8192u && fr.height <= 8192u;
+#pragma unroll
+  for(int k = 0; k < 3; k++)
+    ok = ok && X[k] >= -8191 && X[k] <= 8191 && Y[k] >= -8191 && Y[k] <= 8191;
+  return ok;
+}
+
+// Tiles of the bbox's tile rectangle that may hold a covered pixel: a tile is dropped when one of
+// the three edge functions of the coverage test (va_covered) is negative at every pixel of the
+// tile's part of the bbox, i.e. when its maximum over that pixel rectangle (at a corner chosen by
+// the signs of its x and y coefficients) is negative. Exact for triangles passing
+// va_small_coords, so a dropped tile never holds a covered pixel.
+VA_INLINE i32 va_edge_max(i32 a, i32 b, i32 c, i32 xa, i32 xb, i32 ya, i32 yb)
+{
+  return c + (a >= 0 ? a * xb : a * xa) + (b >= 0 ? b * yb : b * ya);
+}
+
+// 32-bit arithmetic: within va_small_coords the coefficients are below 2^15 in magnitude and
+// every product, constant term and partial sum stays below 8.1e8 (< 2^31).
+VA_INLINE u64 va_tile_mask(const u32 *r, i32 minx, i32 miny, i32 maxx, i32 maxy, u32 tx0, u32 ty0, u32 nx, u32 ny,
+                           u32 tile_shift)
+{
+  const i32 X0 = (i32)r[VA_R_X0], Y0 = (i32)r[VA_R_Y0];
+  const i32 ABX = (i32)r[VA_R_ABX], ABY = (i32)r[VA_R_ABY], ACX = (i32)r[VA_R_ACX], ACY = (i32)r[VA_R_ACY];
+  const i32 AR = (i32)r[VA_R_AREA];
+  // ux = ACX (Y0 - y) - ACY (X0 - x), uy = ABY (X0 - x) - ABX (Y0 - y), b0 = AR - ux - uy
+  const i32 au = ACY, bu = -ACX, cu = ACX * Y0 - ACY * X0;
+  const i32 av = -ABY, bv = ABX, cv = ABY * X0 - ABX * Y0;
+  const i32 a0 = -(au + av), b0 = -(bu + bv), c0 = AR - cu - cv;
+  const i32 T = 1 << tile_shift;
+  u64 mask = 0ull;
+  for(u32 j = 0; j < ny; j++)
+  {
+    const i32 ya = max(miny, (i32)(ty0 + j) * T), yb = min(maxy, (i32)(ty0 + j + 1u) * T) - 1;
+    for(u32 i = 0; i < nx; i++)
+    {
+      const i32 xa = max(minx, (i32)(tx0 + i) * T), xb = min(maxx, (i32)(tx0 + i + 1u) * T) - 1;
+      const bool keep = va_edge_max(au, bu, cu, xa, xb, ya, yb) >= 0 && va_edge_max(av, bv, cv, xa, xb, ya, yb) >= 0 &&
+                        va_edge_max(a0, b0, c0, xa, xb, ya, yb) >= 0;
+      if(keep)
+        mask |= 1ull << (j * nx + i);
+    }
+  }
+  return mask;
+}
+
 // One workgroup covers 256 primitives of a single draw (block_draw maps workgroup -> draw), so
 // the draw's state (push constants, vertex layout, descriptors) is read with scalar loads.
 // Records are assembled in registers, staged in LDS as 16-byte groups (group q of thread t at
 // s_rec4[q * 257 + t], padded against bank conflicts) and written out as the block's contiguous
 // record range with coalesced 16-byte stores.
-// emit_info[g] = (first tile key, tiles per bbox row | frame tiles per row << 16) for va_emit.
+// emit_info[g] = (first tile key, tiles per bbox row | frame tiles per row << 16, tile mask) for
+// va_emit: bit i of the mask = tile i of the bbox's tile rectangle (row-major) may hold a covered
+// pixel; all ones = every tile of the rectangle.
 // Workgroup 0 also zeroes the count-scan's last element and stamps the start of the pass.
 extern "C" __global__ __launch_bounds__(256) void va_setup(
     const VaDraw *__restrict__ draws, const u32 *__restrict__ block_draw, const VaFrame *__restrict__ frames,
     u32 tile_shift, u32 *__restrict__ rec, u32 *__restrict__ tile_count, u64 *__restrict__ part,
-    uint2 *__restrict__ emit_info, u32 total_tris, u64 *__restrict__ stamps)
+    uint4 *__restrict__ emit_info, u32 total_tris, u64 *__restrict__ stamps)
 {
   constexpr u32 Q = VA_REC_STRIDE / 4u;
   __shared__ uint4 s_rec4[Q * 257];  // 16-byte group q of thread t's record at [q * 257 + t]
@@ -221,8 +274,16 @@ extern "C" __global__ __launch_bounds__(256) void va_setup(
             s_rec4[q * 257u + threadIdx.x] = make_uint4(r[4 * q], r[4 * q + 1], r[4 * q + 2], r[4 * q + 3]);
           const u32 tx0 = (u32)minx >> tile_shift, tx1 = (u32)(maxx - 1) >> tile_shift;
           const u32 ty0 = (u32)miny >> tile_shift, ty1 = (u32)(maxy - 1) >> tile_shift;
-          cnt = (tx1 - tx0 + 1u) * (ty1 - ty0 + 1u);
-          emit_info[g] = make_uint2(fr.tile_base + ty0 * fr.tiles_x + tx0, (tx1 - tx0 + 1u) | (fr.tiles_x << 16));
+          const u32 nx = tx1 - tx0 + 1u, ny = ty1 - ty0 + 1u;
+          cnt = nx * ny;
+          u64 mask = ~0ull;
+          if(cnt <= 64u && va_small_coords(X, Y, fr))
+          {
+            mask = va_tile_mask(r, minx, miny, maxx, maxy, tx0, ty0, nx, ny, tile_shift);
+            cnt = (u32)__popcll(mask);
+          }
+          emit_info[g] = make_uint4(fr.tile_base + ty0 * fr.tiles_x + tx0, nx | (fr.tiles_x << 16), (u32)mask,
+                                    (u32)(mask >> 32));
         }
       }
     }
@@ -249,6 +310,23 @@ extern "C" __global__ __launch_bounds__(256) void va_setup(
   va_block_partials(part + (u64)blockIdx.x * 4u, n_in, n_out, n_blocks, n_tested);
 }
 
+// Position of the k-th set bit of m (k < popcount(m)).
+VA_INLINE u32 va_nth_bit(u64 m, u32 k)
+{
+  u32 pos = 0;
+#pragma unroll
+  for(u32 w = 32u; w > 0u; w >>= 1)
+  {
+    const u32 c = (u32)__popcll((m >> pos) & ((1ull << w) - 1ull));
+    if(k >= c)
+    {
+      k -= c;
+      pos += w;
+    }
+  }
+  return pos;
+}
+
 // Pair generation, load-balanced: workgroup b expands primitives [256 b, 256 b + 256). offsets
 // is the exclusive scan of the per-primitive tile counts (total_tris + 1 entries, the last one
 // the pair total); the group's offsets and emit descriptors are staged in LDS, and each thread
@@ -256,11 +334,11 @@ extern "C" __global__ __launch_bounds__(256) void va_setup(
 // offsets. Pairs at or beyond cap are not written (the host sees the total and grows the
 // buffers). Workgroup 0 stamps the start of binning.
 extern "C" __global__ __launch_bounds__(256) void va_emit(
-    const uint2 *__restrict__ emit_info, const u32 *__restrict__ offsets, u32 total_tris, u32 cap,
+    const uint4 *__restrict__ emit_info, const u32 *__restrict__ offsets, u32 total_tris, u32 cap,
     u32 *__restrict__ keys, u32 *__restrict__ vals, u64 *__restrict__ stamps)
 {
   __shared__ u32 s_off[257];
-  __shared__ uint2 s_info[256];
+  __shared__ uint4 s_info[256];
   const u32 t0 = blockIdx.x * 256u;
   if(blockIdx.x == 0u && threadIdx.x == 0u)
     stamps[1] = (u64)wall_clock64();
@@ -281,8 +359,11 @@ extern "C" __global__ __launch_bounds__(256) void va_emit(
     for(u32 step = 128u; step > 0u; step >>= 1)
       if(s_off[i + step] <= p)
         i += step;
-    const u32 k = p - s_off[i];
-    const uint2 e = s_info[i];
+    const uint4 e = s_info[i];
+    const u64 mask = (u64)e.z | ((u64)e.w << 32);
+    // the k-th pair of the primitive is the k-th tile of its mask (the k-th tile of the
+    // rectangle when the mask is all ones)
+    const u32 k = mask == ~0ull ? p - s_off[i] : va_nth_bit(mask, p - s_off[i]);
     const u32 nx = e.y & 0xffffu, tiles_x = e.y >> 16;
     const u32 row = k / nx;
     keys[p] = e.x + row * tiles_x + (k - row * nx);
