@@ -235,3 +235,38 @@ def test_resubmitted_recording_and_descriptor_update(ctx):
     assert np.array_equal(r.read_color(), refrast.render(s2)["color"])
     r.render(draws=s.draws[:3])
     assert np.array_equal(r.read_color(), refrast.render(s2, draws=s2.draws[:3])["color"])
+
+
+@pytest.mark.parametrize("tile", [8, 16, 32])
+def test_slivers_and_far_vertices(ctx, tile):
+    """Thin slivers across many tiles (tile masks drop the tiles they miss) and triangles with
+    vertices far off screen, beyond the window-coordinate range where masks apply (every bbox tile
+    kept): bit-exact either way, at every tile size."""
+    rng = scenes.Rng(77)
+    n = 300
+    pos = np.zeros((3 * n, 4), np.float32)
+    base = np.stack([rng.uniform(n, -1.0, 1.0), rng.uniform(n, -1.0, 1.0)], 1)
+    ang = rng.uniform(n, 0.0, 6.283)
+    length = rng.uniform(n, 0.2, 1.6)
+    width = rng.uniform(n, 0.001, 0.02)
+    d = np.stack([np.cos(ang), np.sin(ang)], 1) * length[:, None]
+    nrm = np.stack([-np.sin(ang), np.cos(ang)], 1) * width[:, None]
+    tri = np.stack([base, base + d, base + d * 0.5 + nrm], 1)  # (n, 3, 2)
+    far = np.arange(n) % 5 == 0                                 # every 5th: one vertex far away
+    tri[far, 2] = tri[far, 2] * 300.0                            # window coordinates ~ +-20000
+    pos[:, :2] = tri.reshape(-1, 2)
+    pos[:, 2] = np.repeat(rng.uniform(n, 0.0, 1.0), 3)
+    pos[:, 3] = 1.0
+    col = rng.bytes(3 * n * 4).reshape(3 * n, 4)
+    pipes = {"main": scenes.Pipeline("vs_mvp_pos4_col", "fs_color")}
+    s = scenes.Scene("slivers", 160, 120, scenes.interleave_pos_col(pos, col), 20, scenes.VATTRS_POS_COL, pipes,
+                     [scenes.Draw("main", 3 * n, 0, False, scenes.push_block(scenes.IDENTITY))])
+    ref = refrast.render(s)
+    assert ref["counters"]["pixels_written"] > 0
+    r = render.SceneRenderer(ctx, s)
+    ctx.set_mode(batched=True, tile=tile)
+    r.render()
+    assert np.array_equal(r.read_color(), ref["color"])
+    assert np.array_equal(r.read_depth().view(np.uint32), ref["depth"].view(np.uint32))
+    st = ctx.stats(0)
+    assert st["pixels_written"] == ref["counters"]["pixels_written"]
