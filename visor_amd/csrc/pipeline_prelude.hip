@@ -95,35 +95,39 @@ VA_INLINE float va_ubo_f(const VaDraw &d, u32 slot, u64 off) { return va_u2f(va_
 // R8G8B8A8_UNORM divides each byte by 255.
 VA_INLINE void va_vattr(const VaDraw &d, u32 vtx, u32 loc, float out[4])
 {
-  out[0] = 0.0f;
-  out[1] = 0.0f;
-  out[2] = 0.0f;
-  out[3] = 1.0f;
   const u32 vb = d.vattr_vb[loc];
   const u64 off = d.vattr_off[loc] + (u64)d.vattr_stride[loc] * vtx;
   const u32 fmt = d.vattr_fmt[loc];
-  const u64 need = (fmt == 109u || fmt == 107u || fmt == 108u) ? 16u : (fmt == 106u || fmt == 104u || fmt == 105u) ? 12u
+  const u32 need = (fmt == 109u || fmt == 107u || fmt == 108u) ? 16u : (fmt == 106u || fmt == 104u || fmt == 105u) ? 12u
                    : (fmt == 103u || fmt == 101u || fmt == 102u) ? 8u : 4u;
-  // an attribute past the end of the bound buffer reads the defaults instead of faulting
-  if(d.vb[vb] == 0ull || off + need > d.vb_bytes[vb]) return;
-  const u32 *w = (const u32 *)((const u8 *)d.vb[vb] + off);
+  // an attribute past the end of the bound buffer reads the defaults instead of faulting; its
+  // loads go to a readable stand-in (the draw's push-constant bytes) and their values are
+  // dropped, so the loads need no branch and can be issued together with the others
+  const bool ok = d.vb[vb] != 0ull && off + need <= d.vb_bytes[vb];
+  const u32 *w = ok ? (const u32 *)((const u8 *)d.vb[vb] + off) : (const u32 *)d.push;
+  const u32 w0 = w[0];
+  const u32 w1 = need >= 8u ? w[1] : 0u;
+  const u32 w2 = need >= 12u ? w[2] : 0u;
+  const u32 w3 = need >= 16u ? w[3] : 0u;
+  float o[4] = {0.0f, 0.0f, 0.0f, 1.0f};
   switch(fmt)
   {
-    case 109: case 107: case 108: out[3] = va_u2f(w[3]);
-    case 106: case 104: case 105: out[2] = va_u2f(w[2]);
-    case 103: case 101: case 102: out[1] = va_u2f(w[1]);
-    case 100: case 98: case 99: out[0] = va_u2f(w[0]); break;
+    case 109: case 107: case 108: o[3] = va_u2f(w3);
+    case 106: case 104: case 105: o[2] = va_u2f(w2);
+    case 103: case 101: case 102: o[1] = va_u2f(w1);
+    case 100: case 98: case 99: o[0] = va_u2f(w0); break;
     case 37:
-    {
-      u32 u = w[0];
-      out[0] = va_unorm8((u & 0x000000ffu) >> 0);
-      out[1] = va_unorm8((u & 0x0000ff00u) >> 8);
-      out[2] = va_unorm8((u & 0x00ff0000u) >> 16);
-      out[3] = va_unorm8((u & 0xff000000u) >> 24);
+      o[0] = va_unorm8((w0 & 0x000000ffu) >> 0);
+      o[1] = va_unorm8((w0 & 0x0000ff00u) >> 8);
+      o[2] = va_unorm8((w0 & 0x00ff0000u) >> 16);
+      o[3] = va_unorm8((w0 & 0xff000000u) >> 24);
       break;
-    }
     default: break;
   }
+  out[0] = ok ? o[0] : 0.0f;
+  out[1] = ok ? o[1] : 0.0f;
+  out[2] = ok ? o[2] : 0.0f;
+  out[3] = ok ? o[3] : 1.0f;
 }
 
 // ---------------------------------------------------------------------------------------------
