@@ -38,17 +38,27 @@ def raster_bytes(st, width, height, color_bytes, texel_bytes):
             + width * height * (color_bytes + 4))
 
 
-def profiled_traffic(tile):
-    """HBM bytes per raster launch from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary of
-    this workload and tile size (null when no such summary exists)."""
+def profiled_value(tile, key):
+    """A per-launch figure of the raster kernel from the committed rocprofv3 summary of this
+    workload and tile size (None when no such summary exists)."""
     path = os.path.join(ROOT, "profiles", "va_raster_%d__256draws_1080p.md" % tile)
     try:
         for line in open(path):
-            if line.startswith("traffic_bytes_per_launch:"):
-                return int(line.split(":")[1])
+            if line.startswith(key + ":"):
+                return int(float(line.split(":")[1]))
     except OSError:
         pass
     return None
+
+
+def profiled_traffic(tile):
+    """HBM bytes per raster launch (FETCH_SIZE / WRITE_SIZE, calibrated), or None."""
+    return profiled_value(tile, "traffic_bytes_per_launch")
+
+
+# VALU issue: 256 CUs x 4 SIMDs, each issuing one wave64 VALU instruction per 4 cycles (16 lanes)
+# at 2.4 GHz; this is the non-packed FP32 rate (78.6 TFLOP/s, half the published packed 157.3).
+VALU_PEAK_GINST = 256 * 4 * 2.4 / 4
 
 
 def serve(ctx, args):
@@ -162,6 +172,14 @@ def main():
     achieved = b_raster / (raster_ms * 1e-3) / 1e9
     roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": profiled_traffic(tile)}
+    # the bound the raster kernel actually meets: wave-level VALU instruction issue
+    # (SQ_INSTS_VALU per launch from the committed summary, over the live kernel time)
+    valu = profiled_value(tile, "valu_insts_per_launch")
+    valu_roofline = None
+    if valu:
+        got = valu / (raster_ms * 1e-3) / 1e9
+        valu_roofline = {"bound": "valu_issue", "achieved": round(got, 1), "peak": VALU_PEAK_GINST,
+                         "unit": "G wave-instructions/s", "frac": round(got / VALU_PEAK_GINST, 4)}
 
     # plain per-draw path
     plain = None
@@ -222,6 +240,7 @@ def main():
                       "bin": round(acc["bin_ms"] / max(args.steps, 1), 4)},
         "graph_replays": acc["graph_launches"],
         "roofline": roofline,
+        "valu_roofline": valu_roofline,
         "cpu_baseline": cpu,
         "served": served,
     }
