@@ -728,6 +728,18 @@ void Queue::RecordTuning(uint64_t sig, const TileConfig &cfg, double ms)
 
 VkResult Queue::RunPass(const std::vector<Op *> &passes, bool fused_clear, ExecStats &st)
 {
+  // A pass that stops part way may leave the finish kernel's accumulators and ticket mid-count:
+  // every early return makes the next pass zero them again.
+  struct ResetOnEarlyReturn
+  {
+    void *&zeroed;
+    bool done = false;
+    ~ResetOnEarlyReturn()
+    {
+      if(!done)
+        zeroed = nullptr;
+    }
+  } reset_guard{stamps_zeroed};
   // a single pass decoded from a recording reuses the tables built for it last time
   const bool reusable = passes.size() == 1 && passes[0]->key != 0;
   const bool have = reusable && tables_cache.key == passes[0]->key &&
@@ -1124,6 +1136,7 @@ VkResult Queue::RunPass(const std::vector<Op *> &passes, bool fused_clear, ExecS
   }
 
   drained = true;  // everything enqueued so far has completed, ev_end marks its end
+  reset_guard.done = true;
   st.raster_launches++;
   st.total_tris += tri_total;
   st.total_pairs += total_pairs;
