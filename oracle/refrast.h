@@ -6,9 +6,11 @@
  * It is the checker the GPU path is compared against and the timed CPU baseline. Only tests,
  * the smoke check and the benchmark's CPU-baseline leg may load it; the product never does.
  *
- * The reference itself cannot be built here (it needs the MSVC dialect and CRT, and LLVM 6 for
- * its shader JIT), so every function below cites the reference lines it restates, and the tests
- * pin it with closed forms, brute force and special cases.
+ * The reference's raster path cannot be built here (it needs the MSVC dialect and CRT, and LLVM 6
+ * for its shader JIT), so every function below cites the reference lines it restates, and the
+ * tests pin it with closed forms, brute force and special cases. The one part that builds from its
+ * own source, the vendored BC2/BC3 decoder (3rdparty/decompress.c), is compiled into oracle/_ref
+ * and checks rr_decode_bc_block directly.
  */
 #ifndef VISOR_REFRAST_H
 #define VISOR_REFRAST_H
