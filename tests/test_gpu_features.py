@@ -270,3 +270,59 @@ def test_slivers_and_far_vertices(ctx, tile):
     assert np.array_equal(r.read_depth().view(np.uint32), ref["depth"].view(np.uint32))
     st = ctx.stats(0)
     assert st["pixels_written"] == ref["counters"]["pixels_written"]
+
+
+def test_buffer_to_image_regions(ctx):
+    """vkCmdCopyBufferToImage with image offsets, a source row longer than the region
+    (bufferRowLength), a mip level and two layers at once writes exactly those texels of the
+    linear image layout; a region past the image edge is rejected."""
+    import ctypes as C
+
+    from visor_amd import vk
+    W, H, mips, layers = 20, 12, 2, 3
+    img, mem, view, size = ctx.image(W, H, vk.FMT["R8G8B8A8_UNORM"], mips, layers, host_visible=True)
+    lvl = [W * H * 4, (W // 2) * (H // 2) * 4]
+    slice_bytes = sum(lvl)
+    assert size == slice_bytes * layers
+    C.memset(ctx.map(mem, size), 0, size)
+    rng = np.random.default_rng(4)
+    src = rng.integers(1, 256, 4096, dtype=np.uint8)
+    stage, smem = ctx.buffer(src.size, host_visible=True)
+    C.memmove(ctx.map(smem, src.size), src.tobytes(), src.size)
+    want = np.zeros(size, np.uint8)
+
+    def region(off, row, rows, mip, layer0, nlayers, x, y, w, h, expect=True):
+        r = vk.BufferImageCopy(bufferOffset=off, bufferRowLength=row, bufferImageHeight=rows,
+                               imageSubresource=vk.ImageSubresourceLayers(aspectMask=1, mipLevel=mip,
+                                                                          baseArrayLayer=layer0, layerCount=nlayers),
+                               imageOffset=vk.Offset3D(x=x, y=y, z=0), imageExtent=vk.Extent3D(width=w, height=h, depth=1))
+        if not expect:
+            return r
+        lw = max(1, W >> mip)
+        rr, rh = row or w, rows or h
+        for layer in range(nlayers):
+            base = (layer0 + layer) * slice_bytes + sum(lvl[:mip])
+            for j in range(h):
+                s0 = off + (layer * rr * rh + j * rr) * 4
+                d0 = base + ((y + j) * lw + x) * 4
+                want[d0:d0 + w * 4] = src[s0:s0 + w * 4]
+        return r
+
+    regions = [region(0, 9, 0, 0, 0, 1, 3, 2, 7, 5),      # offset region, longer source rows
+               region(400, 0, 0, 1, 1, 2, 1, 1, 8, 4),    # level 1, layers 1-2
+               region(1000, 11, 6, 0, 2, 1, 12, 6, 8, 6)]  # bufferImageHeight > height
+    cb = ctx.begin()
+    arr = (vk.BufferImageCopy * len(regions))(*regions)
+    ctx.icd.vkCmdCopyBufferToImage(cb, stage, img, 7, len(regions), arr)
+    ctx.end_submit(cb)
+    got = np.frombuffer(C.string_at(ctx.map(mem, size), size), np.uint8)
+    assert np.array_equal(got, want)
+    # past the right edge of level 0: the submit fails and the image is unchanged
+    bad = (vk.BufferImageCopy * 1)(region(0, 0, 0, 0, 0, 1, 15, 0, 8, 2, expect=False))
+    cb = ctx.begin()
+    ctx.icd.vkCmdCopyBufferToImage(cb, stage, img, 7, 1, bad)
+    ctx.icd.vkEndCommandBuffer(cb)
+    assert ctx.submit([cb]) != 0
+    got2 = np.frombuffer(C.string_at(ctx.map(mem, size), size), np.uint8)
+    diff = np.flatnonzero(got2 != want)
+    assert diff.size == 0, (diff.size, diff[:8].tolist(), got2[diff[:8]].tolist(), want[diff[:8]].tolist())

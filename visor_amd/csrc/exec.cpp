@@ -551,7 +551,10 @@ bool Queue::Decode(const VkCommandBuffer_T *cb, Submission &sub, std::string &er
         op.copy.dstImg = d->dstImage;
         op.copy.dstBuf = nullptr;
         op.copy.bi = d->region;
+        // the copy writes only its region: the rest of the destination's host-visible memory
+        // must reach the device first, or the download after the submission would replace it
         AddMem(sub.uploads, d->srcBuffer->mem);
+        AddMem(sub.uploads, d->dstImage->mem);
         AddMem(sub.downloads, d->dstImage->mem);
         sub.ops.push_back(op);
         break;
@@ -567,6 +570,7 @@ bool Queue::Decode(const VkCommandBuffer_T *cb, Submission &sub, std::string &er
         op.copy.dstImg = nullptr;
         op.copy.bb = d->region;
         AddMem(sub.uploads, d->srcBuffer->mem);
+        AddMem(sub.uploads, d->dstBuffer->mem);
         AddMem(sub.downloads, d->dstBuffer->mem);
         sub.ops.push_back(op);
         break;
@@ -583,20 +587,46 @@ VkResult Queue::RunCopy(const CopyDesc &c)
 {
   if(c.to_image)
   {
-    // reference cmd_exec.cpp:143-173: whole mip, tightly packed, one layer
+    // reference cmd_exec.cpp:143-173 copies whole, tightly packed levels of one layer; here any
+    // region of an uncompressed image (offsets, bufferRowLength / bufferImageHeight, several
+    // layers) is copied as 2D strided DMA, one layer at a time. Block-compressed images keep
+    // the whole-level rule (their linear layout has no rows of texels).
     const VkImage_T *img = c.dstImg;
-    uint32_t mip = c.bi.imageSubresource.mipLevel;
-    uint64_t w = std::max(1u, img->extent.width >> mip), h = std::max(1u, img->extent.height >> mip);
-    if(c.bi.bufferRowLength != 0 || c.bi.bufferImageHeight != 0 || c.bi.imageOffset.x || c.bi.imageOffset.y ||
-       c.bi.imageOffset.z || c.bi.imageExtent.width != w || c.bi.imageExtent.height != h ||
-       c.bi.imageSubresource.layerCount != 1)
+    const VkBufferImageCopy &r = c.bi;
+    const uint32_t mip = r.imageSubresource.mipLevel;
+    const uint64_t W = std::max(1u, img->extent.width >> mip), H = std::max(1u, img->extent.height >> mip);
+    const uint64_t bpp = img->bytesPerPixel;
+    const bool block = img->format == VK_FORMAT_BC2_UNORM_BLOCK || img->format == VK_FORMAT_BC3_UNORM_BLOCK;
+    const uint64_t x = (uint32_t)r.imageOffset.x, y = (uint32_t)r.imageOffset.y;
+    const uint64_t w = r.imageExtent.width, h = r.imageExtent.height;
+    const uint64_t row = r.bufferRowLength ? r.bufferRowLength : w;
+    const uint64_t rows = r.bufferImageHeight ? r.bufferImageHeight : h;
+    const uint32_t layers = r.imageSubresource.layerCount, layer0 = r.imageSubresource.baseArrayLayer;
+    bool ok = mip < img->mipLevels && r.imageOffset.x >= 0 && r.imageOffset.y >= 0 && r.imageOffset.z == 0 &&
+              r.imageExtent.depth <= 1 && x + w <= W && y + h <= H && row >= w && rows >= h && layers >= 1 &&
+              (uint64_t)layer0 + layers <= img->arrayLayers;
+    if(ok && block)
+      ok = x == 0 && y == 0 && w == W && h == H && row == w && rows == h;
+    const uint64_t layer_bytes = row * rows * bpp;
+    const uint64_t src_need = r.bufferOffset + layer_bytes * (layers - 1) + ((h - 1) * row + w) * bpp;
+    if(ok && w && h)
+      ok = src_need <= c.src->size && c.src->offset + src_need <= c.src->mem->size &&
+           img->offset + SubresourceOffset(img, mip, layer0 + layers - 1) +
+                                                               ((y + h - 1) * W + x + w) * bpp <= img->mem->size;
+    if(!ok)
     {
-      LogError("vkCmdCopyBufferToImage: only whole, tightly packed single-layer mips are supported");
+      LogError("vkCmdCopyBufferToImage: region outside the image or the buffer, or a partial copy of a "
+               "block-compressed image");
       return VK_ERROR_FEATURE_NOT_PRESENT;
     }
-    uint64_t offs = SubresourceOffset(img, mip, c.bi.imageSubresource.baseArrayLayer);
-    HIPCHK(hipMemcpyAsync(img->mem->dev + img->offset + offs, c.src->mem->dev + c.src->offset + c.bi.bufferOffset,
-                          w * h * img->bytesPerPixel, hipMemcpyDeviceToDevice, stream));
+    if(!w || !h)
+      return VK_SUCCESS;
+    for(uint32_t l = 0; l < layers; l++)
+    {
+      uint8_t *dst = img->mem->dev + img->offset + SubresourceOffset(img, mip, layer0 + l) + (y * W + x) * bpp;
+      const uint8_t *src = c.src->mem->dev + c.src->offset + r.bufferOffset + layer_bytes * l;
+      HIPCHK(hipMemcpy2DAsync(dst, W * bpp, src, row * bpp, w * bpp, h, hipMemcpyDeviceToDevice, stream));
+    }
   }
   else
   {
