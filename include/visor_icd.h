@@ -59,6 +59,7 @@ typedef struct VhStats {
   uint64_t tile_size;       /* launch shape of the last pass: tile side (8, 16, 32) ...      */
   uint64_t chunk;           /* ... bin entries staged in LDS per round ...                  */
   uint64_t persistent;      /* ... and 1 when the raster grid was the persistent form       */
+  uint64_t tuning;          /* 1 when the last pass ran one of the autotuner's trial shapes */
 } VhStats;
 
 /* 1 when a HIP device is usable. */

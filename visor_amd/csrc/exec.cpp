@@ -1149,6 +1149,7 @@ VkResult Queue::RunPass(const std::vector<Op *> &passes, bool fused_clear, ExecS
   memcpy(&c.tris_in, h_out + 2, 8 * sizeof(uint64_t));  // tris_in .. shaded_dwrite
   c.draws = ndraws;
   st.last_cfg = cfg;
+  st.last_tuning = tuning;
   {
     // stamps: 0 setup start, 1 binning start, 2 raster start, 3 finish start (constant-rate clock)
     const uint64_t s0 = h_out[10], s1 = h_out[11], s2 = h_out[12], s3 = h_out[13];

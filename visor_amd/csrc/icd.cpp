@@ -1338,6 +1338,7 @@ int vh_get_stats(VkQueue queue, int which, VhStats *out)
   out->tile_size = 1ull << s.last_cfg.tile_shift;
   out->chunk = s.last_cfg.chunk;
   out->persistent = s.last_cfg.persistent;
+  out->tuning = s.last_tuning ? 1u : 0u;
   return 0;
 }
 

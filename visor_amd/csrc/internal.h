@@ -356,6 +356,7 @@ struct ExecStats
   uint64_t total_tris = 0, total_pairs = 0, total_tiles = 0;
   uint64_t graph_launches = 0;  // passes replayed from a captured graph
   TileConfig last_cfg;          // launch shape of the last pass
+  bool last_tuning = false;     // the last pass ran one of the autotuner's trial shapes
   VaCounters ctr{};
 };
 

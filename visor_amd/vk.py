@@ -238,7 +238,7 @@ class VhStats(C.Structure):
                 ("tris_in", u64), ("tris_out", u64), ("blocks", u64), ("pixels_tested", u64), ("pixels_written", u64),
                 ("depth_passed", u64), ("shaded_blend", u64), ("shaded_dwrite", u64),
                 ("graph_launches", u64), ("tile_size", u64), ("chunk", u64),
-                ("persistent", u64)]
+                ("persistent", u64), ("tuning", u64)]
 
 
 class VhServerConfig(C.Structure):
