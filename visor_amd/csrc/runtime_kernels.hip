@@ -197,7 +197,9 @@ __global__ __launch_bounds__(256) void va_sort_hist_kernel(const u32 *__restrict
 
 // Wave w of the workgroup ranks pairs [base + 1024 w, base + 1024 (w + 1)) in order, 64 at a time:
 // lanes with equal digits find each other with eight ballots, and per-wave digit counters in LDS
-// carry the rank across steps. Waves are then ordered by a per-digit prefix over the waves.
+// carry the rank across steps. Waves are then ordered by a per-digit prefix over the waves. The
+// ranked pairs are staged in LDS in sorted order (keys, then values through the same buffer)
+// and stored from there, consecutive threads to consecutive addresses within each digit's run.
 __global__ __launch_bounds__(256) void va_sort_scatter_kernel(const u32 *__restrict__ kin, const u32 *__restrict__ vin,
                                                               u32 *__restrict__ kout, u32 *__restrict__ vout,
                                                               const u32 *__restrict__ n_dev, u32 cap, u32 shift,
@@ -205,6 +207,9 @@ __global__ __launch_bounds__(256) void va_sort_scatter_kernel(const u32 *__restr
 {
   __shared__ u32 s_cnt[4][256];
   __shared__ u32 s_goff[256];
+  __shared__ u32 s_dstart[256];
+  __shared__ u32 s_stage[RK_SORT_B];
+  __shared__ u32 s_w[4];
   const u32 tid = threadIdx.x, lane = tid & 63u, w = tid >> 6;
   const u32 n = min(*n_dev, cap);
   const u32 base = blockIdx.x * RK_SORT_B;
@@ -243,27 +248,54 @@ __global__ __launch_bounds__(256) void va_sort_scatter_kernel(const u32 *__restr
     rank[j] = valid ? cur + r : 0xffffffffu;
   }
   __syncthreads();
-  {
-    u32 run = 0;
+  u32 dcount = 0;  // this workgroup's pairs with digit tid
 #pragma unroll
-    for(u32 k = 0; k < 4u; k++)
+  for(u32 k = 0; k < 4u; k++)
+  {
+    const u32 c = s_cnt[k][tid];
+    s_cnt[k][tid] = dcount;
+    dcount += c;
+  }
+  // The workgroup's pairs in sorted order are staged in LDS, so that the global stores go out
+  // in runs of consecutive addresses: local index i with digit d goes to i + (goff[d] - start[d]).
+  u32 nvalid;
+  const u32 dstart = rk_block_exclusive<4>(dcount, s_w, nvalid);
+  s_dstart[tid] = dstart;
+  s_goff[tid] -= dstart;
+  __syncthreads();
+#pragma unroll
+  for(u32 j = 0; j < RK_SORT_I; j++)
+    if(rank[j] != 0xffffffffu)
     {
-      const u32 c = s_cnt[k][tid];
-      s_cnt[k][tid] = run;
-      run += c;
+      const u32 d = (key[j] >> shift) & 255u;
+      rank[j] += s_dstart[d] + s_cnt[w][d];  // now the local sorted index
+      s_stage[rank[j]] = key[j];
+    }
+  __syncthreads();
+  u32 gpos[RK_SORT_I];
+#pragma unroll
+  for(u32 j = 0; j < RK_SORT_I; j++)
+  {
+    const u32 i = j * RK_SORT_T + tid;
+    if(i < nvalid)
+    {
+      const u32 k = s_stage[i];
+      gpos[j] = i + s_goff[(k >> shift) & 255u];
+      kout[gpos[j]] = k;
     }
   }
   __syncthreads();
 #pragma unroll
   for(u32 j = 0; j < RK_SORT_I; j++)
-  {
     if(rank[j] != 0xffffffffu)
-    {
-      const u32 d = (key[j] >> shift) & 255u;
-      const u32 pos = s_goff[d] + s_cnt[w][d] + rank[j];
-      kout[pos] = key[j];
-      vout[pos] = val[j];
-    }
+      s_stage[rank[j]] = val[j];
+  __syncthreads();
+#pragma unroll
+  for(u32 j = 0; j < RK_SORT_I; j++)
+  {
+    const u32 i = j * RK_SORT_T + tid;
+    if(i < nvalid)
+      vout[gpos[j]] = s_stage[i];
   }
 }
 
