@@ -153,9 +153,10 @@ VA_INLINE u64 va_tile_mask(const u32 *r, i32 minx, i32 miny, i32 maxx, i32 maxy,
 
 // One workgroup covers 256 primitives of a single draw (block_draw maps workgroup -> draw), so
 // the draw's state (push constants, vertex layout, descriptors) is read with scalar loads.
-// Records are assembled in registers, staged in LDS as 16-byte groups (group q of thread t at
-// s_rec4[q * 257 + t], padded against bank conflicts) and written out as the block's contiguous
-// record range with coalesced 16-byte stores.
+// Records are assembled in registers and stored by their own thread as 16-byte groups: one
+// 128-byte record is one whole cache line, which the L2 merges before it goes to HBM. (Staging
+// the workgroup's records in LDS for wave-contiguous stores costs 33 KB of LDS, which halves the
+// resident workgroups, and measured slower.)
 // emit_info[g] = (first tile key, tiles per bbox row | frame tiles per row << 16, tile mask) for
 // va_emit: bit i of the mask = tile i of the bbox's tile rectangle (row-major) may hold a covered
 // pixel; all ones = every tile of the rectangle.
@@ -166,8 +167,6 @@ extern "C" __global__ __launch_bounds__(256) void va_setup(
     uint4 *__restrict__ emit_info, u32 total_tris, u64 *__restrict__ stamps)
 {
   constexpr u32 Q = VA_REC_STRIDE / 4u;
-  __shared__ uint4 s_rec4[Q * 257];  // 16-byte group q of thread t's record at [q * 257 + t]
-  __shared__ u32 s_keep[256];        // 1 for threads whose primitive produced a record
   if(blockIdx.x == 0u && threadIdx.x == 0u)
   {
     stamps[0] = (u64)wall_clock64();
@@ -271,7 +270,7 @@ extern "C" __global__ __launch_bounds__(256) void va_setup(
           }
 #pragma unroll
           for(u32 q = 0; q < Q; q++)
-            s_rec4[q * 257u + threadIdx.x] = make_uint4(r[4 * q], r[4 * q + 1], r[4 * q + 2], r[4 * q + 3]);
+            ((uint4 *)(rec + (u64)g * VA_REC_STRIDE))[q] = make_uint4(r[4 * q], r[4 * q + 1], r[4 * q + 2], r[4 * q + 3]);
           const u32 tx0 = (u32)minx >> tile_shift, tx1 = (u32)(maxx - 1) >> tile_shift;
           const u32 ty0 = (u32)miny >> tile_shift, ty1 = (u32)(maxy - 1) >> tile_shift;
           const u32 nx = tx1 - tx0 + 1u, ny = ty1 - ty0 + 1u;
@@ -288,22 +287,6 @@ extern "C" __global__ __launch_bounds__(256) void va_setup(
       }
     }
     tile_count[g] = cnt;
-  }
-  s_keep[threadIdx.x] = n_tested != 0ull ? 1u : 0u;
-  // the block's records: triangles [prim0, prim0 + nvalid) of the draw, contiguous from g0;
-  // slots of primitives without a record (culled, degenerate, empty bbox) are not written
-  __syncthreads();
-  {
-    const u32 prim0 = (blockIdx.x - d.block_base) * 256u;
-    const u32 nvalid = min(256u, d.ntris - prim0);
-    uint4 *dst = (uint4 *)(rec + (u64)(d.tri_base + prim0) * VA_REC_STRIDE);
-    const u32 nq = nvalid * Q;
-    for(u32 i = threadIdx.x; i < nq; i += 256u)
-    {
-      const u32 t = i / Q, q = i - t * Q;
-      if(s_keep[t])
-        dst[i] = s_rec4[q * 257u + t];
-    }
   }
   // counters: one row of partial sums per workgroup (no contended atomics), columns in
   // VaCounters order from tris_in
