@@ -871,6 +871,20 @@ VA_INLINE void va_raster_tile(const VaDraw *__restrict__ draws, const VaFrame *_
   va_block_partials(row, n_cov, n_pass, n_blend, n_dw);
 }
 
+// Workgroups are dispatched round-robin over the 8 XCDs (workgroup b to XCD b % 8), each with its
+// own L2. Workgroup b takes tile va_xcd_tile(b, n), a bijection of [0, n) that deals runs of 8
+// consecutive tiles to the XCDs in turn (the tail past the last whole round keeps b), so
+// neighbouring tiles of a run, which share triangles, read those records through the same L2,
+// while every XCD still gets work from everywhere on the target.
+VA_INLINE u32 va_xcd_tile(u32 b, u32 n)
+{
+  const u32 full = n & ~63u;
+  if(b >= full)
+    return b;
+  const u32 x = b & 7u, s = b >> 3;  // s-th workgroup of XCD x
+  return (((s >> 3) << 3) + x) * 8u + (s & 7u);
+}
+
 // pair_total / pair_cap: when the pairs overflowed the bins the launch writes nothing, so the
 // host can grow the buffers and run the pass again on an untouched target. Workgroup 0 stamps
 // the start of the raster stage.
@@ -881,8 +895,8 @@ VA_INLINE void va_raster_tile(const VaDraw *__restrict__ draws, const VaFrame *_
       const u32 *__restrict__ bin_end, u32 ntiles, u32 use_fused_clear, u64 *__restrict__ part,            \
       const u32 *__restrict__ pair_total, u32 pair_cap, u64 *__restrict__ stamp)                            \
   {                                                                                                         \
-    const u32 tile = blockIdx.x;                                                                            \
-    if(tile == 0u && threadIdx.x == 0u)                                                                     \
+    const u32 tile = va_xcd_tile(blockIdx.x, ntiles);                                                       \
+    if(blockIdx.x == 0u && threadIdx.x == 0u)                                                               \
       *stamp = (u64)wall_clock64();                                                                         \
     if(tile < ntiles && *pair_total <= pair_cap)                                                            \
       va_raster_tile<T>(draws, frames, nframes, rec, bins, bin_start, bin_end, use_fused_clear, part, tile); \
