@@ -872,17 +872,18 @@ VA_INLINE void va_raster_tile(const VaDraw *__restrict__ draws, const VaFrame *_
 }
 
 // Workgroups are dispatched round-robin over the 8 XCDs (workgroup b to XCD b % 8), each with its
-// own L2. Workgroup b takes tile va_xcd_tile(b, n), a bijection of [0, n) that deals runs of 8
+// own L2. Workgroup b takes tile va_xcd_tile(b, n), a bijection of [0, n) that deals runs of 16
 // consecutive tiles to the XCDs in turn (the tail past the last whole round keeps b), so
 // neighbouring tiles of a run, which share triangles, read those records through the same L2,
-// while every XCD still gets work from everywhere on the target.
+// while every XCD still gets work from everywhere on the target (one contiguous band per XCD
+// unbalances the load).
 VA_INLINE u32 va_xcd_tile(u32 b, u32 n)
 {
-  const u32 full = n & ~63u;
+  const u32 full = n & ~127u;
   if(b >= full)
     return b;
   const u32 x = b & 7u, s = b >> 3;  // s-th workgroup of XCD x
-  return (((s >> 3) << 3) + x) * 8u + (s & 7u);
+  return (((s >> 4) << 3) + x) * 16u + (s & 15u);
 }
 
 // pair_total / pair_cap: when the pairs overflowed the bins the launch writes nothing, so the
