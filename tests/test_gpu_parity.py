@@ -65,6 +65,31 @@ def test_many_draws_small_bit_exact(ctx, tile):
     assert out["stats"]["depth_passed"] == ref["counters"]["depth_passed"]
 
 
+@pytest.mark.parametrize("seed", range(20, 28))
+def test_many_draws_seeds_odd_sizes_bit_exact(ctx, seed):
+    """Seeded variants on a 333x197 target, which no tile size divides: partial edge tiles, and
+    tile counts with a remainder past the last whole round of per-XCD tile runs (1050 tiles at
+    8x8, 273 at 16x16, 77 at 32x32); the tile size cycles with the seed."""
+    tile = (8, 16, 32)[seed % 3]
+    s = scenes.many_draws(ndraws=16, width=333, height=197, tex_size=32, seed=seed, max_tris=1500)
+    ref = refrast.render(s)
+    out = _render_gpu(ctx, s, tile=tile)
+    _compare(s, out, ref)
+    st, c = out["stats"], ref["counters"]
+    assert (st["pixels_written"], st["depth_passed"], st["tris_out"]) == (
+        c["pixels_written"], c["depth_passed"], c["tris_out"])
+
+
+@pytest.mark.parametrize("seed", range(2, 8))
+def test_random_triangles_seeds_bit_exact(ctx, seed):
+    """Seeded random clip-space triangles (large, overlapping, both windings) on odd widths."""
+    s = scenes.random_triangles(ntris=800, width=257 + seed, height=131, seed=seed)
+    ref = refrast.render(s)
+    out = _render_gpu(ctx, s, tile=(8, 16, 32)[seed % 3])
+    _compare(s, out, ref)
+    assert out["stats"]["pixels_written"] == ref["counters"]["pixels_written"]
+
+
 def test_micro_triangles_small_bit_exact(ctx):
     """RGBA16F target, micro-triangles, 8 trilinear taps of an array texture, additive draws."""
     s = scenes.micro_triangles(ntris=20000, ndraws=4, width=256, height=256, tex_size=64, layers=4)
