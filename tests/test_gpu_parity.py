@@ -90,6 +90,26 @@ def test_random_triangles_seeds_bit_exact(ctx, seed):
     assert out["stats"]["pixels_written"] == ref["counters"]["pixels_written"]
 
 
+def test_bin_overflow_reruns_the_pass(ctx):
+    """2000 large random triangles on 640x480 with 8x8 tiles make about 2.5M (tile, triangle)
+    pairs, over twice the first bin capacity of a new queue (max(2^20, 2 x triangles)): the first
+    try's raster writes nothing, the pass runs again with larger bins, and the target equals the
+    oracle's (so the failed try left it untouched). A second submit runs at the grown capacity."""
+    fresh = render.Context()  # a queue whose bin capacity has not grown yet
+    s = scenes.random_triangles(ntris=2000, width=640, height=480, seed=9)
+    ref = refrast.render(s)
+    r = render.SceneRenderer(fresh, s)
+    fresh.set_mode(batched=True, tile=8)
+    r.render()
+    first = {"color": r.read_color(), "depth": r.read_depth(), "stats": fresh.stats(0)}
+    assert first["stats"]["bin_pairs"] > 1 << 20
+    _compare(s, first, ref)
+    assert first["stats"]["pixels_written"] == ref["counters"]["pixels_written"]
+    r.render()
+    assert np.array_equal(r.read_color(), first["color"])
+    assert np.array_equal(r.read_depth().view(np.uint32), first["depth"].view(np.uint32))
+
+
 def test_micro_triangles_small_bit_exact(ctx):
     """RGBA16F target, micro-triangles, 8 trilinear taps of an array texture, additive draws."""
     s = scenes.micro_triangles(ntris=20000, ndraws=4, width=256, height=256, tex_size=64, layers=4)
