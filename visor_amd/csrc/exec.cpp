@@ -1008,7 +1008,7 @@ VkResult Queue::RunPass(const std::vector<Op *> &passes, bool fused_clear, ExecS
           uint32_t *keys = (uint32_t *)d_keys, *vals = (uint32_t *)d_vals;
           uint4 *einfo = (uint4 *)d_einfo;
           void *eargs[] = {&einfo, &offs, &tri_total, &cap, &keys, &vals, &stamps};
-          VA_TRY(hipModuleLaunchKernel(mod->emit, (tri_total + 255) / 256, 1, 1, 256, 1, 1, 0, stream, eargs, nullptr));
+          VA_TRY(hipModuleLaunchKernel(mod->emit, (tri_total + 1023) / 1024, 1, 1, 1024, 1, 1, 0, stream, eargs, nullptr));
           int alt = 0;
           VA_TRY(va_sort_pairs((uint32_t *)d_stemp, keys, vals, (uint32_t *)d_keys2, (uint32_t *)d_vals2, total_dev, cap,
                                BitsFor(tile_total), &alt, stream));
