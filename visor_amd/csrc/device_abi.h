@@ -16,6 +16,7 @@
 #define VA_MAX_UBO 4
 #define VA_MAX_LEVELS 16
 #define VA_REC_HDR 17   /* header words of a triangle record, varyings follow */
+#define VA_SETUP_PRIMS 256u /* primitives (threads) per setup workgroup (512: no faster) */
 
 /* record header word indices */
 #define VA_R_X0 0
@@ -68,7 +69,7 @@ typedef struct VaDraw {
   unsigned int blend_enable, src_factor, dst_factor, blend_op;
   unsigned int frame;      /* index into the VaFrame table */
   unsigned int slot;       /* shader pair slot inside the pipeline module */
-  unsigned int block_base; /* first setup workgroup of this draw (256 primitives each) */
+  unsigned int block_base; /* first setup workgroup of this draw (VA_SETUP_PRIMS primitives each) */
   unsigned int pad;
   unsigned char push[128];
   unsigned long long ubo[VA_MAX_UBO];

@@ -837,7 +837,7 @@ VkResult Queue::RunPass(const std::vector<Op *> &passes, bool fused_clear, ExecS
         v.tri_base = tri_total;
         tri_total += v.ntris;
         v.block_base = (uint32_t)block_draw.size();
-        for(uint32_t b = 0; b < (v.ntris + 255) / 256; b++)
+        for(uint32_t b = 0; b < (v.ntris + VA_SETUP_PRIMS - 1) / VA_SETUP_PRIMS; b++)
           block_draw.push_back((uint32_t)draws.size());
         draws.push_back(v);
       }
@@ -992,7 +992,7 @@ VkResult Queue::RunPass(const std::vector<Op *> &passes, bool fused_clear, ExecS
           // the setup kernel zeroes the scan's last count and writes stamp 0 itself
           uint4 *einfo = (uint4 *)d_einfo;
           void *args[] = {&dd, &bd, &df, (void *)&ts, &rec, &cnt, &part, &einfo, &tri_total, &stamps};
-          VA_TRY(hipModuleLaunchKernel(mod->setup, nblocks, 1, 1, 256, 1, 1, 0, stream, args, nullptr));
+          VA_TRY(hipModuleLaunchKernel(mod->setup, nblocks, 1, 1, VA_SETUP_PRIMS, 1, 1, 0, stream, args, nullptr));
           VA_TRY(va_scan_u32((uint32_t *)d_temp, cnt, offs, tri_total + 1, stream));
         }
         else

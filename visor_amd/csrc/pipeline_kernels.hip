@@ -151,8 +151,9 @@ VA_INLINE u64 va_tile_mask(const u32 *r, i32 minx, i32 miny, i32 maxx, i32 maxy,
   return mask;
 }
 
-// One workgroup covers 256 primitives of a single draw (block_draw maps workgroup -> draw), so
-// the draw's state (push constants, vertex layout, descriptors) is read with scalar loads.
+// One workgroup covers VA_SETUP_PRIMS primitives of a single draw (block_draw maps workgroup ->
+// draw), so the draw's state (push constants, vertex layout, descriptors) is read with scalar
+// loads.
 // Records are assembled in registers and stored by their own thread as 16-byte groups: one
 // 128-byte record is one whole cache line, which the L2 merges before it goes to HBM. (Staging
 // the workgroup's records in LDS for wave-contiguous stores costs 33 KB of LDS, which halves the
@@ -161,7 +162,7 @@ VA_INLINE u64 va_tile_mask(const u32 *r, i32 minx, i32 miny, i32 maxx, i32 maxy,
 // va_emit: bit i of the mask = tile i of the bbox's tile rectangle (row-major) may hold a covered
 // pixel; all ones = every tile of the rectangle.
 // Workgroup 0 also zeroes the count-scan's last element and stamps the start of the pass.
-extern "C" __global__ __launch_bounds__(256) void va_setup(
+extern "C" __global__ __launch_bounds__(VA_SETUP_PRIMS) void va_setup(
     const VaDraw *__restrict__ draws, const u32 *__restrict__ block_draw, const VaFrame *__restrict__ frames,
     u32 tile_shift, u32 *__restrict__ rec, u32 *__restrict__ tile_count, u64 *__restrict__ part,
     uint4 *__restrict__ emit_info, u32 total_tris, u64 *__restrict__ stamps)
@@ -174,7 +175,7 @@ extern "C" __global__ __launch_bounds__(256) void va_setup(
   }
   const u32 di = __builtin_amdgcn_readfirstlane(block_draw[blockIdx.x]);
   const VaDraw &d = draws[di];
-  const u32 prim = (blockIdx.x - d.block_base) * 256u + threadIdx.x;
+  const u32 prim = (blockIdx.x - d.block_base) * VA_SETUP_PRIMS + threadIdx.x;
   const u32 g = d.tri_base + prim;
   u64 n_in = 0, n_out = 0, n_tested = 0, n_blocks = 0;
   if(prim < d.ntris)
