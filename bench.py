@@ -108,19 +108,7 @@ def main():
                          "capacity makes every client resubmit as soon as its frame is done")
     ap.add_argument("--serve-seconds", type=float, default=3.0)
     ap.add_argument("--serve-clients", type=int, default=64)
-    ap.add_argument("--calib-copy", action="store_true",
-                    help="run only a 1 GiB device copy (2 GiB of HBM traffic) to calibrate FETCH_SIZE/WRITE_SIZE")
     args = ap.parse_args()
-    if args.calib_copy:
-        import torch
-        a = torch.ones(1 << 28, dtype=torch.float32, device="cuda")
-        b = torch.empty_like(a)
-        for _ in range(3):
-            b.copy_(a)
-        torch.cuda.synchronize()
-        print(json.dumps({"calib_copy_bytes_read": a.numel() * 4, "calib_copy_bytes_written": b.numel() * 4,
-                          "launches": 3}))
-        return
     if args.gpus != 1:
         print("bench.py measures one GPU: every box here has one MI355X; --gpus %d is not supported" % args.gpus,
               file=sys.stderr)

@@ -78,9 +78,16 @@ int vh_set_tile_size(VkQueue queue, int tile);
 /* 0 = vkQueueSubmit returns after the GPU finished (the reference's behaviour), 1 = returns at
  * once; completion through fences / vkQueueWaitIdle. */
 int vh_set_async_submit(VkQueue queue, int on);
-/* which = 0: last submission; 1: accumulated since vh_reset_stats. */
+/* which = 0: last submission; 1: accumulated since vh_reset_stats; 2: the frame the last
+ * vkQueuePresentKHR ended (submissions between the two latest presents); 3: the current frame
+ * (submissions since the last present). A present ends and begins frame statistics as the
+ * reference's does (wsi.cpp:215-217, stats.cpp:25-41). */
 int vh_get_stats(VkQueue queue, int which, VhStats *out);
 int vh_reset_stats(VkQueue queue);
+/* The headless present's output (the reference blits the presented image to a window,
+ * wsi.cpp:201-213): index of the image the last vkQueuePresentKHR showed, the number of
+ * presents, and the host address of that image's bytes (NULL before the first present). */
+int vh_swapchain_frame(VkSwapchainKHR swapchain, uint32_t *image_index, uint64_t *presents, const void **pixels);
 /* SPIR-V -> HIP C of one entry point "main" (stage 0 vertex, 4 fragment); returns the length
  * written, or -1 with the error text in out. Needs no GPU. */
 int vh_translate_spirv(const uint32_t *code, size_t nwords, int stage, char *out, size_t out_size);

@@ -89,6 +89,7 @@ typedef enum VkResult {
   VK_ERROR_INCOMPATIBLE_DRIVER = -9,
   VK_ERROR_TOO_MANY_OBJECTS = -10,
   VK_ERROR_FORMAT_NOT_SUPPORTED = -11,
+  VK_ERROR_OUT_OF_DATE_KHR = -1000001004,
   VK_RESULT_MAX_ENUM = 0x7FFFFFFF
 } VkResult;
 

@@ -561,6 +561,13 @@ static void pc_float(const rr_state *st, uint32_t off, float *out, int n)
   memcpy(out, st->pushconsts + off, 4 * n);
 }
 
+/* Uniform-buffer floats of set 0, binding `binding`: reference spirv_compile.cpp:552-558
+ * (GetDescriptorBufferPointer: the bound buffer's bytes at the descriptor's offset). */
+static void ubo_float(const rr_state *st, uint32_t binding, uint32_t off, float *out, int n)
+{
+  memcpy(out, st->ubo[binding] + off, 4 * n);
+}
+
 /* reference spirv_compile.cpp:423-431 (Float4x4TimesVec4), column-major m[col*4+row] */
 static void mat4_times_vec4(const float *m, const float v[4], float out[4])
 {
@@ -605,6 +612,16 @@ static void vs_program(const rr_state *st, uint32_t vtx, rr_vertex *o)
       rr_get_vertex_attribute(st, vtx, 0, pos);
       rr_get_vertex_attribute(st, vtx, 1, col);
       mat4_times_vec4(mvp, pos, o->position);
+      store_interp(o, 0, col, 4);
+      break;
+    }
+    case RR_VS_UBO_POS4_COL:
+    {
+      float pos[4], col[4], umvp[16];
+      rr_get_vertex_attribute(st, vtx, 0, pos);
+      rr_get_vertex_attribute(st, vtx, 1, col);
+      ubo_float(st, RR_UBO_BINDING, 0, umvp, 16);
+      mat4_times_vec4(umvp, pos, o->position);
       store_interp(o, 0, col, 4);
       break;
     }
@@ -693,6 +710,16 @@ static void fs_program(const rr_state *st, float pixdepth, const float b3[3][4],
     case RR_FS_TEX_ONLY:
       tex_uv(st, 0, b3, tri, 1, out);
       break;
+    case RR_FS_UBO_TINT:
+    {
+      /* out = in0 * tint + bias, component-wise (OpFMul then OpFAdd, two roundings) */
+      float tint[4], bias[4];
+      ubo_float(st, RR_UBO_BINDING, 64, tint, 4);
+      ubo_float(st, RR_UBO_BINDING, 80, bias, 4);
+      for(int c = 0; c < 4; c++)
+        out[c] = interp(b, tri, 0, c) * tint[c] + bias[c];
+      break;
+    }
     case RR_FS_CUBE_ENV:
       sample_cube_full(st->tex[0], st->samp[0], interp(b, tri, 0, 0), interp(b, tri, 0, 1), interp(b, tri, 0, 2), out);
       break;
@@ -760,14 +787,30 @@ typedef struct tri_work {
   int minx, miny, maxx, maxy;
 } tri_work;
 
+/* int32 subtraction and negation wrapping modulo 2^32, as the reference's x64 build computes
+ * them (far-off vertices reach the wrapping range; C leaves signed overflow undefined). */
+static inline int rr_wsub(int a, int b) { return (int)((uint32_t)a - (uint32_t)b); }
+static inline int rr_wneg(int a) { return (int)(0u - (uint32_t)a); }
+
+/* float -> int32 as the reference's x64 build converts (cvttss2si: truncation toward zero, and
+ * the "integer indefinite" 0x80000000 for NaN and every value outside [-2^31, 2^31)). C leaves
+ * the out-of-range case undefined, so it is spelled out here. */
+int32_t rr_f2i(float f)
+{
+  if(f >= -2147483648.0f && f < 2147483648.0f)
+    return (int32_t)f;
+  return INT32_MIN;
+}
+
 /* reference rasterizer.cpp:239-253 (ToWindow): each op is one FP32 rounding, left to right. */
 void rr_to_window(uint32_t w, uint32_t h, const float p[4], int32_t out[2])
 {
-  out[0] = (int32_t)((p[0] / p[3] + 1.0f) * 0.5f * (float)w);
-  out[1] = (int32_t)((p[1] * -1.0f / p[3] + 1.0f) * 0.5f * (float)h);
+  out[0] = rr_f2i((p[0] / p[3] + 1.0f) * 0.5f * (float)w);
+  out[1] = rr_f2i((p[1] * -1.0f / p[3] + 1.0f) * 0.5f * (float)h);
 }
 
-/* reference rasterizer.cpp:100-119 (GetIndex; vertexOffset is not applied) */
+/* reference rasterizer.cpp:100-119 (GetIndex). The draw's vertexOffset is recorded
+ * (cmd_record.cpp:125-135, st->vertex_offset here) but GetIndex never adds it. */
 static uint32_t get_index(const rr_state *st, uint32_t i, int indexed)
 {
   if(!indexed)
@@ -849,8 +892,8 @@ static size_t shade_verts(const rr_state *st, int num_verts, uint32_t first, int
  * (barycentric) and :545-588. Returns coverage; writes the normalised (pre-perspective) depth. */
 static int bary_at(const tri_work *w, int x, int y, float out[4], float *pixdepth)
 {
-  const int PAx = w->tri[0] - x;
-  const int PAy = w->tri[1] - y;
+  const int PAx = rr_wsub(w->tri[0], x);
+  const int PAy = rr_wsub(w->tri[1], y);
   const int ux = (int)((uint32_t)w->ACx * (uint32_t)PAy - (uint32_t)w->ACy * (uint32_t)PAx);
   const int uy = (int)((uint32_t)PAx * (uint32_t)w->ABy - (uint32_t)PAy * (uint32_t)w->ABx);
   int b0 = (int)((uint32_t)w->area2 - ((uint32_t)ux + (uint32_t)uy));
@@ -1016,10 +1059,10 @@ void rr_clear_color(rr_image *t, const float rgba[4])
     return;
   }
   uint8_t e[4];
-  e[2] = (uint8_t)(int)(rgba[0] * 255.0f);
-  e[1] = (uint8_t)(int)(rgba[1] * 255.0f);
-  e[0] = (uint8_t)(int)(rgba[2] * 255.0f);
-  e[3] = (uint8_t)(int)(rgba[3] * 255.0f);
+  e[2] = (uint8_t)rr_f2i(rgba[0] * 255.0f);
+  e[1] = (uint8_t)rr_f2i(rgba[1] * 255.0f);
+  e[0] = (uint8_t)rr_f2i(rgba[2] * 255.0f);
+  e[3] = (uint8_t)rr_f2i(rgba[3] * 255.0f);
   if(t->bpp == 1)
   {
     memset(t->pixels, e[2], n);
@@ -1049,15 +1092,15 @@ static void draw_setup(const rr_state *st, int num_verts, uint32_t first, int in
     const int32_t *t = &win[2 * i];
     const rr_vertex *vo = &sv[i];
     tris_in++;
-    int area2 = (int)((uint32_t)(t[2] - t[0]) * (uint32_t)(t[5] - t[1]) -
-                      (uint32_t)(t[3] - t[1]) * (uint32_t)(t[4] - t[0]));
+    int area2 = (int)(((uint32_t)t[2] - (uint32_t)t[0]) * ((uint32_t)t[5] - (uint32_t)t[1]) -
+                      ((uint32_t)t[3] - (uint32_t)t[1]) * ((uint32_t)t[4] - (uint32_t)t[0]));
     if(area2 == 0)
       continue;
     int flipped = area2, barymul = 1;
     if(st->pipeline->front_face == RR_FRONT_CW)
     {
       barymul = -barymul;
-      flipped = -flipped;
+      flipped = rr_wneg(flipped);
     }
     if(flipped > 0 && (st->pipeline->cull_mode & RR_CULL_FRONT))
       continue;
@@ -1066,7 +1109,7 @@ static void draw_setup(const rr_state *st, int num_verts, uint32_t first, int in
       if(st->pipeline->cull_mode & RR_CULL_BACK)
         continue;
       barymul = -barymul;
-      flipped = -flipped;
+      flipped = rr_wneg(flipped);
     }
     tris_out++;
     int minx = t[0], miny = t[1], maxx = t[0], maxy = t[1];
@@ -1083,10 +1126,10 @@ static void draw_setup(const rr_state *st, int num_verts, uint32_t first, int in
     if(maxy > (int)h - 1) maxy = (int)h - 1;
     tri_work wk;
     wk.state = st;
-    wk.ABx = t[2] - t[0];
-    wk.ABy = t[3] - t[1];
-    wk.ACx = t[4] - t[0];
-    wk.ACy = t[5] - t[1];
+    wk.ABx = rr_wsub(t[2], t[0]);
+    wk.ABy = rr_wsub(t[3], t[1]);
+    wk.ACx = rr_wsub(t[4], t[0]);
+    wk.ACy = rr_wsub(t[5], t[1]);
     wk.vsout = vo;
     wk.tri = t;
     wk.barymul = barymul;

@@ -66,13 +66,19 @@ enum {
 enum {
   RR_VS_MVP_POS4_COL = 1,     /* gl_Position = pc.mvp * pos; out0 = col               */
   RR_VS_MVP_NORMAL_UV = 2,    /* gl_Position = pc.mvp * vec4(pos,1); out0 = pc.nmat*nrm; out1 = uv */
+  RR_VS_UBO_POS4_COL = 3,     /* gl_Position = ubo.mvp * pos; out0 = col               */
   RR_FS_COLOR = 101,          /* out = in0                                            */
   RR_FS_LAMBERT_TEX = 102,    /* normalize, max(dot), texture(uv) * light, alpha 1    */
   RR_FS_LAMBERT_TEX_A = 103,  /* as above with alpha = pc.alpha (push constant 112)  */
   RR_FS_MULTITAP8 = 104,      /* 8 array-texture taps accumulated in FP32             */
   RR_FS_TEX_ONLY = 105,       /* out = texture(tex, in1.xy)                           */
   RR_FS_CUBE_ENV = 106,       /* out = texture(samplerCube, in0.xyz)                  */
+  RR_FS_UBO_TINT = 107,       /* out = in0 * ubo.tint + ubo.bias                      */
 };
+
+/* Uniform block of the UBO programs: set 0, binding RR_UBO_BINDING; mat4 mvp at 0 (column-major),
+ * vec4 tint at 64, vec4 bias at 80. */
+enum { RR_UBO_BINDING = 1 };
 
 typedef struct rr_image {
   uint8_t *pixels;          /* linear rows; mips packed within a layer, then layers     */
@@ -109,6 +115,8 @@ typedef struct rr_state {
   const rr_image *tex[4];   /* set 0, binding i                                         */
   const rr_sampler *samp[4];
   uint8_t pushconsts[128];
+  const uint8_t *ubo[4];    /* set 0, binding i: uniform buffer bytes at the descriptor's offset */
+  int32_t vertex_offset;    /* vkCmdDrawIndexed's vertexOffset: recorded, not applied     */
 } rr_state;
 
 typedef struct rr_counters {
@@ -133,6 +141,7 @@ void rr_draw_threaded(const rr_state *state, int num_verts, uint32_t first, int 
                       int nthreads, rr_counters *ctr);
 
 /* Exposed pieces, for the pinning tests. */
+int32_t rr_f2i(float f);
 void rr_to_window(uint32_t w, uint32_t h, const float pos[4], int32_t out[2]);
 void rr_sample(const rr_image *img, const rr_sampler *samp, const float coord[3],
                const float ddx[2], const float ddy[2], int use_lod, float out[4]);

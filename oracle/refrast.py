@@ -4,6 +4,7 @@ restatement of the reference rasterizer (refrast.c).
 Only tests, the smoke check and the benchmark's CPU-baseline leg import this module.
 """
 
+import contextlib
 import ctypes as C
 import os
 import time
@@ -39,7 +40,7 @@ class Pipeline(C.Structure):
 class State(C.Structure):
     _fields_ = [("ib", vp), ("index_type", u32), ("vbs", vp * 4), ("col0", C.POINTER(Image)), ("depth", C.POINTER(Image)),
                 ("pipeline", C.POINTER(Pipeline)), ("tex", C.POINTER(Image) * 4), ("samp", C.POINTER(Sampler) * 4),
-                ("pushconsts", C.c_uint8 * 128)]
+                ("pushconsts", C.c_uint8 * 128), ("ubo", vp * 4), ("vertex_offset", C.c_int32)]
 
 
 class Counters(C.Structure):
@@ -50,33 +51,51 @@ class Counters(C.Structure):
 _lib = None
 
 
+def _bind(path):
+    l = C.CDLL(path)
+    l.rr_clear_color.argtypes = [C.POINTER(Image), C.POINTER(f32)]
+    l.rr_clear_depth.argtypes = [C.POINTER(Image), f32]
+    l.rr_draw.argtypes = [C.POINTER(State), C.c_int, u32, C.c_int, C.POINTER(Counters)]
+    l.rr_draw_threaded.argtypes = [C.POINTER(State), C.c_int, u32, C.c_int, C.c_int, C.POINTER(Counters)]
+    l.rr_to_window.argtypes = [u32, u32, C.POINTER(f32), C.POINTER(C.c_int32)]
+    l.rr_sample.argtypes = [C.POINTER(Image), C.POINTER(Sampler), C.POINTER(f32), C.POINTER(f32), C.POINTER(f32),
+                            C.c_int, C.POINTER(f32)]
+    l.rr_sample_tex_wrapped.argtypes = [f32, f32, C.POINTER(Image), u64, C.POINTER(f32)]
+    l.rr_sample_cube_wrapped.argtypes = [f32, f32, f32, C.POINTER(Image), C.POINTER(f32)]
+    l.rr_sample_cube.argtypes = [C.POINTER(Image), C.POINTER(Sampler), C.POINTER(f32), C.POINTER(f32)]
+    l.rr_decode_bc_block.argtypes = [u32, C.POINTER(C.c_uint8), C.POINTER(C.c_uint8)]
+    l.rr_log2.argtypes = [f32]
+    l.rr_log2.restype = f32
+    l.rr_f32_to_f16.argtypes = [f32]
+    l.rr_f32_to_f16.restype = C.c_uint16
+    l.rr_f16_to_f32.argtypes = [C.c_uint16]
+    l.rr_f16_to_f32.restype = f32
+    l.rr_subresource_offset.argtypes = [C.POINTER(Image), u32, u32]
+    l.rr_subresource_offset.restype = u64
+    l.rr_f2i.argtypes = [f32]
+    l.rr_f2i.restype = C.c_int32
+    return l
+
+
 def lib():
     global _lib
     if _lib is None:
         if not os.path.exists(LIB):
             raise RuntimeError("oracle library not built (make -C oracle): " + LIB)
-        l = C.CDLL(LIB)
-        l.rr_clear_color.argtypes = [C.POINTER(Image), C.POINTER(f32)]
-        l.rr_clear_depth.argtypes = [C.POINTER(Image), f32]
-        l.rr_draw.argtypes = [C.POINTER(State), C.c_int, u32, C.c_int, C.POINTER(Counters)]
-        l.rr_draw_threaded.argtypes = [C.POINTER(State), C.c_int, u32, C.c_int, C.c_int, C.POINTER(Counters)]
-        l.rr_to_window.argtypes = [u32, u32, C.POINTER(f32), C.POINTER(C.c_int32)]
-        l.rr_sample.argtypes = [C.POINTER(Image), C.POINTER(Sampler), C.POINTER(f32), C.POINTER(f32), C.POINTER(f32),
-                                C.c_int, C.POINTER(f32)]
-        l.rr_sample_tex_wrapped.argtypes = [f32, f32, C.POINTER(Image), u64, C.POINTER(f32)]
-        l.rr_sample_cube_wrapped.argtypes = [f32, f32, f32, C.POINTER(Image), C.POINTER(f32)]
-        l.rr_sample_cube.argtypes = [C.POINTER(Image), C.POINTER(Sampler), C.POINTER(f32), C.POINTER(f32)]
-        l.rr_decode_bc_block.argtypes = [u32, C.POINTER(C.c_uint8), C.POINTER(C.c_uint8)]
-        l.rr_log2.argtypes = [f32]
-        l.rr_log2.restype = f32
-        l.rr_f32_to_f16.argtypes = [f32]
-        l.rr_f32_to_f16.restype = C.c_uint16
-        l.rr_f16_to_f32.argtypes = [C.c_uint16]
-        l.rr_f16_to_f32.restype = f32
-        l.rr_subresource_offset.argtypes = [C.POINTER(Image), u32, u32]
-        l.rr_subresource_offset.restype = u64
-        _lib = l
+        _lib = _bind(LIB)
     return _lib
+
+
+@contextlib.contextmanager
+def use_library(path):
+    """Runs the block with another build of refrast.c (the mutation tests' planted mistakes)."""
+    global _lib
+    saved = _lib
+    _lib = _bind(path)
+    try:
+        yield _lib
+    finally:
+        _lib = saved
 
 
 def decode_bc_block(fmt, block):
@@ -106,6 +125,7 @@ def render(scene, threads=0, draws=None, time_only=False):
     dep_img = make_image(depth, s.width, s.height, fmt=126)
     vdata = np.frombuffer(s.vertex_data, dtype=np.uint8)
     idata = np.frombuffer(s.index_data, dtype=np.uint8) if s.index_data is not None else None
+    ubo_arrays = [np.frombuffer(u, dtype=np.uint8) for u in s.uniforms]
     tex_arrays, tex_imgs, samplers = [], [], []
     for t in s.textures:
         a = np.frombuffer(t.data, dtype=np.uint8)
@@ -142,6 +162,9 @@ def render(scene, threads=0, draws=None, time_only=False):
             st.tex[0] = C.pointer(tex_imgs[d.texture])
             st.samp[0] = C.pointer(samplers[d.texture])
         C.memmove(st.pushconsts, d.push, 128)
+        st.vertex_offset = d.vertex_offset
+        if d.uniform is not None:
+            st.ubo[shaders.UBO_BINDING] = ubo_arrays[d.uniform].ctypes.data
         if threads > 0:
             L.rr_draw_threaded(C.byref(st), d.count, d.first, 1 if d.indexed else 0, threads, C.byref(ctr))
         else:

@@ -21,7 +21,7 @@ L = refrast.lib()
 
 def to_window(W, H, p):
     out = (C.c_int32 * 2)()
-    L.rr_to_window(W, H, (C.c_float * 4)(*p), out)
+    refrast.lib().rr_to_window(W, H, (C.c_float * 4)(*p), out)
     return out[0], out[1]
 
 
@@ -36,6 +36,47 @@ def test_to_window_special_values_and_truncation():
     assert to_window(256, 256, (-1.01, 0, 0, 1))[0] == -1  # -1.28 -> -1
     # perspective divide by w
     assert to_window(256, 256, (2, 0, 0, 2)) == (256, 128)
+
+
+INT_MIN = -2 ** 31
+
+
+def test_float_to_int_is_x86_truncation():
+    """Float-to-int32 as the reference's x64 build converts (cvttss2si): truncation toward zero
+    inside [-2^31, 2^31), and 0x80000000 for NaN, infinities and every value outside (C leaves
+    that case undefined; a saturating conversion, which is what a GPU does natively, gives
+    INT_MAX or 0 there). It decides far-off vertices (a vertex with w near 0, for which the
+    reference does no clipping) and out-of-range clear values."""
+    f2i = refrast.lib().rr_f2i
+    for x, want in [(0.0, 0), (1.9, 1), (-1.9, -1), (2147483520.0, 2147483520), (-2147483648.0, INT_MIN),
+                    (2147483648.0, INT_MIN), (1e12, INT_MIN), (-1e12, INT_MIN), (float("inf"), INT_MIN),
+                    (float("-inf"), INT_MIN), (float("nan"), INT_MIN)]:
+        assert f2i(x) == want, x
+    # ToWindow of a vertex at w = 1e-7: x/w = 5e6 -> window 6.4e8 (in range); x = 100: out of range
+    assert to_window(256, 256, (0.5, -0.5, 0, 1e-7)) == (640000128, 640000128)
+    assert to_window(256, 256, (100.0, -100.0, 0, 1e-7)) == (INT_MIN, INT_MIN)
+    assert to_window(256, 256, (float("nan"), 0.0, 0, 1.0))[0] == INT_MIN
+    # the colour clear stores the low byte of the converted value (reference rasterizer.cpp:341-345)
+    s = tri_scene([[(0, 0), (2, 0), (0, 2)]], W=16, H=16, clear_color=(1e10, 2.0, -0.5, -1e10))
+    # bytes [2]=R: 1e10*255 -> INT_MIN -> 0; [1]=G: 510 -> 254; [0]=B: -127.5 -> -127 -> 129; [3]=A -> 0
+    assert refrast.render(s)["color"][15, 15].tolist() == [129, 254, 0, 0]
+
+
+def test_vertex_offset_is_not_applied():
+    """vkCmdDrawIndexed's vertexOffset is recorded but GetIndex never adds it (reference
+    rasterizer.cpp:100-119, cmd_exec.cpp:129-142): indices 0, 1, 2 with vertexOffset 3 draw
+    vertices 0..2, a right triangle whose coverage has the closed form below, not vertices 3..5."""
+    n = 10
+    s = tri_scene([[(0, 0), (n, 0), (0, n)], [(16, 16), (30, 16), (16, 30)]], W=32, H=32)
+    s.index_data = np.array([0, 1, 2], np.uint32).tobytes()
+    s.index_type = 1
+    s.draws = [scenes.Draw("p", 3, 0, True, s.draws[0].push, vertex_offset=3)]
+    r = refrast.render(s)
+    expect = np.zeros((32, 32), bool)
+    for y in range(n):
+        for x in range(n):
+            expect[y, x] = x + y <= n
+    assert np.array_equal(covered_mask(r["color"]), expect)
 
 
 # ---- coverage: closed form, brute force, bbox rules ---------------------------------------------

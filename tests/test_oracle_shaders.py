@@ -12,7 +12,7 @@ import dataclasses
 import numpy as np
 
 from oracle import refrast
-from visor_amd import scenes
+from visor_amd import scenes, shaders
 from visor_amd.vk import BF, CMP
 
 W = H = 64
@@ -89,6 +89,43 @@ def test_interpolation_matches_float64_barycentrics():
         assert checked > 40
 
 
+def depth_scene(corners, zndc, w):
+    """One triangle at exact pixel corners with per-vertex NDC depth zndc and clip w (x, y and z
+    scaled by w, so the perspective divide returns the NDC values)."""
+    pos = np.zeros((3, 4), np.float32)
+    for k, (px, py) in enumerate(corners):
+        x, y = ndc(px, py)
+        pos[k] = (x * w[k], y * w[k], zndc[k] * w[k], w[k])
+    pipes = {"main": scenes.Pipeline("vs_mvp_pos4_col", "fs_color", depth_op=CMP["ALWAYS"], depth_write=True)}
+    return scenes.Scene("tri", W, H, scenes.interleave_pos_col(pos, np.array(COLOURS, np.uint8)), 20,
+                        scenes.VATTRS_POS_COL, pipes, [scenes.Draw("main", 3, 0, False, scenes.push_block(scenes.IDENTITY))])
+
+
+def test_depth_matches_screen_affine_interpolation():
+    """Stored depth is the screen-space (pre-perspective) barycentric mix of the vertices' z/w
+    (reference rasterizer.cpp:551-558): with sloped z and strongly unequal w it equals the
+    float64 affine interpolation within a few float32 ulps, which a perspective-corrected depth
+    (weights divided by w) misses by orders of magnitude."""
+    corners = [(6, 5), (58, 20), (15, 60)]
+    zndc = np.array([0.1, 0.55, 0.9], np.float32)
+    w = (1.0, 2.5, 0.375)  # exact in binary32, so x * w / w returns x and the corners stay exact
+    r = refrast.render(depth_scene(corners, zndc, w))
+    dep = r["depth"]
+    checked, spread = 0, []
+    for py in range(H):
+        for px in range(W):
+            _, inside = _bary64(corners, (1.0, 1.0, 1.0), px, py)
+            if not inside or px >= 58 or py >= 60:
+                continue
+            lam, _ = _bary64(corners, (1.0, 1.0, 1.0), px, py)  # w = 1: the screen-affine weights
+            want = float(lam @ zndc.astype(np.float64))
+            got = float(dep[py, px])
+            assert abs(got - want) <= 4 * np.spacing(np.float32(1.0)) / 2, (px, py, got, want)
+            checked += 1
+            spread.append(got)
+    assert checked > 1000 and max(spread) - min(spread) > 0.5
+
+
 def test_mvp_translation_moves_the_image_by_whole_pixels():
     """A translation of 8 pixels (0.25 in NDC at 64 px) in the push-constant MVP, column-major as
     the shaders read it, moves colour and coverage by exactly 8 pixels."""
@@ -100,6 +137,34 @@ def test_mvp_translation_moves_the_image_by_whole_pixels():
     moved = refrast.render(color_scene(corners, COLOURS, mvp=mvp))["color"]
     assert np.array_equal(moved[8:, 8:], base[:-8, :-8])
     assert not moved[:8].any() and not moved[:, :8].any()
+
+
+def ubo_scene(corners, colours, mvp, tint, bias):
+    """color_scene drawn with the uniform-buffer programs: MVP, tint and bias from set 0 binding 1."""
+    s = color_scene(corners, colours)
+    s.pipelines = {"main": scenes.Pipeline("vs_ubo_pos4_col", "fs_ubo_tint", depth_op=CMP["ALWAYS"])}
+    s.uniforms = [shaders.ubo_block_bytes(mvp, tint, bias)]
+    s.draws = [dataclasses.replace(s.draws[0], push=b"\0" * 128, uniform=0)]
+    return s
+
+
+def test_uniform_buffer_programs():
+    """The UBO-fed programs read their block through the descriptor (reference
+    spirv_compile.cpp:552-558): with tint 1 and bias 0 they render the push-constant programs'
+    image bit for bit (also under an 8-pixel MVP translation read from the buffer), and a flat
+    colour c comes out as trunc(clamp01(c/255 * tint + bias) * 255) within 1 LSB."""
+    corners = [(6, 5), (40, 14), (12, 44)]
+    mvp = np.eye(4)
+    mvp[0, 3], mvp[1, 3] = 0.25, -0.25
+    for m in (np.eye(4), mvp):
+        want = refrast.render(color_scene(corners, COLOURS, mvp=m))["color"]
+        got = refrast.render(ubo_scene(corners, COLOURS, m, (1, 1, 1, 1), (0, 0, 0, 0)))["color"]
+        assert np.array_equal(got, want)
+    flat = [(200, 100, 40, 255)] * 3
+    tint, bias = (0.5, 0.25, 1.5, 1.0), (0.25, 0.5, -0.125, 0.0)
+    img = refrast.render(ubo_scene(corners, flat, np.eye(4), tint, bias))["color"]
+    want = np.floor(np.clip(np.array(flat[0][:3]) / 255.0 * tint[:3] + bias[:3], 0, 1) * 255)
+    _check_uniform(img, want)
 
 
 def lit_scene(fs, normals, tex_value=(100, 150, 200, 255), nmat=np.eye(3), alpha=1.0, blend=False, layers=1,
@@ -142,6 +207,20 @@ def test_lambert_extremes():
                                  ([(0.8, -0.64, 0.0)] * 3, np.eye(3), 0.25)]:
         img = refrast.render(lit_scene("fs_lambert_tex", normals, nmat=nmat))["color"]
         _check_uniform(img, np.floor(t * scale))
+
+
+def test_lambert_through_a_rotation_normal_matrix():
+    """The normal matrix is applied as M * n with M's columns 16 bytes apart in the push block
+    (reference spirv_compile.cpp:433-441): through a general rotation (not symmetric, so M^T n
+    differs) each lit result equals the float64 Lambert of normalize(M n) within 1 LSB."""
+    t = np.array([100, 150, 200])
+    rot = scenes.rotation(0.4, -0.9, 1.3)
+    for n in [(0.6, 0.0, 0.8), (0.0, -0.28, 0.96), (-0.36, 0.48, 0.8)]:
+        m = rot @ np.array(n)
+        m /= np.linalg.norm(m)
+        light = max(float(m @ LIGHT), 0.0) * 0.75 + 0.25
+        img = refrast.render(lit_scene("fs_lambert_tex", [n] * 3, nmat=rot))["color"]
+        _check_uniform(img, np.floor(t * light))
 
 
 def test_push_constant_alpha_blends():

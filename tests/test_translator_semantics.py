@@ -302,6 +302,23 @@ def test_integer_min_max_abs_not_and_unsigned(lib, tmp_path):
     assert np.array_equal(got, np.float32([-5, 17, ~(-17), 1000 >> 2]))
 
 
+def test_float_to_int_conversions_follow_x64(lib, tmp_path):
+    """ConvertFToS / ConvertFToU of out-of-range values give what the reference's x64 JIT code
+    gives (LLVM fptosi -> cvttss2si: 0x80000000 outside int32 range; fptoui to i32 -> a 64-bit
+    signed conversion whose low 32 bits are kept), not the saturated values of a GPU conversion."""
+    p = FS()
+    m = p.m
+    r = [m.op("ConvertSToF", p.f, m.op("ConvertFToS", p.i, p.x(0, 0))),
+         m.op("ConvertSToF", p.f, m.op("ConvertFToS", p.i, p.x(0, 1))),
+         m.op("ConvertUToF", p.f, m.op("ConvertFToU", p.u, p.x(0, 2))),
+         m.op("ConvertUToF", p.f, m.op("ConvertFToU", p.u, p.x(0, 3)))]
+    got = run_fs_rows(lib, tmp_path, p.finish(r), [[[3e9, -2.5, -1.0, 5e9], [0, 0, 0, 0]],
+                                                    [[-1e12, 2147483520.0, 1e20, 7.9], [0, 0, 0, 0]]])
+    want = np.float32([[-2147483648.0, -2.0, 4294967295.0, 5e9 % 2 ** 32],
+                       [-2147483648.0, 2147483520.0, 0.0, 7.0]])
+    assert np.array_equal(got, want), got
+
+
 def test_compare_select_logic(lib, tmp_path):
     p = FS()
     m = p.m

@@ -569,6 +569,18 @@ bool Queue::Decode(const VkCommandBuffer_T *cb, Submission &sub, std::string &er
         op.copy.dstBuf = d->dstBuffer;
         op.copy.dstImg = nullptr;
         op.copy.bb = d->region;
+        // reference cmd_exec.cpp:174-182 copies without checks; here a region outside either
+        // buffer (or its memory) is refused before any device work
+        auto inside = [](const VkBuffer_T *b, VkDeviceSize off, VkDeviceSize size) {
+          return b && b->mem && off <= b->size && size <= b->size - off && b->offset <= b->mem->size &&
+                 b->size <= b->mem->size - b->offset;
+        };
+        if(!inside(d->srcBuffer, d->region.srcOffset, d->region.size) ||
+           !inside(d->dstBuffer, d->region.dstOffset, d->region.size))
+        {
+          err = "buffer copy region outside its buffers";
+          return false;
+        }
         AddMem(sub.uploads, d->srcBuffer->mem);
         AddMem(sub.uploads, d->dstBuffer->mem);
         AddMem(sub.downloads, d->dstBuffer->mem);
@@ -637,6 +649,36 @@ VkResult Queue::RunCopy(const CopyDesc &c)
 }
 
 // reference rasterizer.cpp:312-361 (ClearTarget x2)
+void AccumulateStats(ExecStats &acc, const ExecStats &st)
+{
+  acc.gpu_ms_total += st.gpu_ms_total;
+  acc.raster_ms += st.raster_ms;
+  acc.setup_ms += st.setup_ms;
+  acc.bin_ms += st.bin_ms;
+  acc.raster_launches += st.raster_launches;
+  acc.total_tris += st.total_tris;
+  acc.total_pairs += st.total_pairs;
+  acc.total_tiles += st.total_tiles;
+  acc.graph_launches += st.graph_launches;
+  acc.ctr.pixels_written += st.ctr.pixels_written;
+  acc.ctr.depth_passed += st.ctr.depth_passed;
+  acc.ctr.tris_in += st.ctr.tris_in;
+  acc.ctr.tris_out += st.ctr.tris_out;
+  acc.ctr.draws += st.ctr.draws;
+  acc.ctr.blocks += st.ctr.blocks;
+  acc.ctr.pixels_tested += st.ctr.pixels_tested;
+  acc.ctr.shaded_blend += st.ctr.shaded_blend;
+  acc.ctr.shaded_dwrite += st.ctr.shaded_dwrite;
+  acc.last_cfg = st.last_cfg;
+  acc.last_tuning = st.last_tuning;
+}
+
+// float -> int32 as x64 converts it (truncation; 0x80000000 for NaN and outside [-2^31, 2^31))
+static int32_t X64FloatToInt(float f)
+{
+  return (f >= -2147483648.0f && f < 2147483648.0f) ? (int32_t)f : INT32_MIN;
+}
+
 VkResult Queue::ClearFrame(const FrameDesc &f)
 {
   if(f.clear_color)
@@ -654,11 +696,12 @@ VkResult Queue::ClearFrame(const FrameDesc &f)
     }
     else
     {
+      // reference rasterizer.cpp:341-345: the low byte of the x64 float->int conversion
       uint8_t e[4];
-      e[2] = (uint8_t)(int)(c[0] * 255.0f);
-      e[1] = (uint8_t)(int)(c[1] * 255.0f);
-      e[0] = (uint8_t)(int)(c[2] * 255.0f);
-      e[3] = (uint8_t)(int)(c[3] * 255.0f);
+      e[2] = (uint8_t)X64FloatToInt(c[0] * 255.0f);
+      e[1] = (uint8_t)X64FloatToInt(c[1] * 255.0f);
+      e[0] = (uint8_t)X64FloatToInt(c[2] * 255.0f);
+      e[3] = (uint8_t)X64FloatToInt(c[3] * 255.0f);
       if(img->bytesPerPixel == 1)
         HIPCHK(hipMemsetD8Async((hipDeviceptr_t)img->devPixels(), e[2], n, stream));
       else
@@ -1257,24 +1300,8 @@ VkResult Queue::Execute(std::vector<Submission> &subs)
     }
     st.gpu_ms_total = ms;
     last_stats = st;
-    accum_stats.gpu_ms_total += st.gpu_ms_total;
-    accum_stats.raster_ms += st.raster_ms;
-    accum_stats.setup_ms += st.setup_ms;
-    accum_stats.bin_ms += st.bin_ms;
-    accum_stats.raster_launches += st.raster_launches;
-    accum_stats.total_tris += st.total_tris;
-    accum_stats.total_pairs += st.total_pairs;
-    accum_stats.total_tiles += st.total_tiles;
-    accum_stats.graph_launches += st.graph_launches;
-    accum_stats.ctr.pixels_written += st.ctr.pixels_written;
-    accum_stats.ctr.depth_passed += st.ctr.depth_passed;
-    accum_stats.ctr.tris_in += st.ctr.tris_in;
-    accum_stats.ctr.tris_out += st.ctr.tris_out;
-    accum_stats.ctr.draws += st.ctr.draws;
-    accum_stats.ctr.blocks += st.ctr.blocks;
-    accum_stats.ctr.pixels_tested += st.ctr.pixels_tested;
-    accum_stats.ctr.shaded_blend += st.ctr.shaded_blend;
-    accum_stats.ctr.shaded_dwrite += st.ctr.shaded_dwrite;
+    AccumulateStats(accum_stats, st);
+    AccumulateStats(frame_stats, st);
     SignalFence(sub.fence, res);
   }
   return res;

@@ -11,6 +11,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <thread>
 
 #include "../../include/visor_icd.h"
 #include "internal.h"
@@ -330,6 +331,23 @@ VKAPI_ATTR VkResult VKAPI_CALL vkWaitForFences(VkDevice, uint32_t n, const VkFen
 {
   auto deadline = std::chrono::steady_clock::now() +
                   std::chrono::nanoseconds(timeout > (uint64_t)INT64_MAX / 2 ? (uint64_t)INT64_MAX / 2 : timeout);
+  if(!waitAll)
+  {
+    // any one fence: poll all of them until one is signalled or the deadline passes
+    for(;;)
+    {
+      for(uint32_t i = 0; i < n; i++)
+        if(VkFence f = fences[i])
+        {
+          std::lock_guard<std::mutex> lk(f->m);
+          if(f->signaled)
+            return f->status;
+        }
+      if(std::chrono::steady_clock::now() >= deadline)
+        return VK_TIMEOUT;
+      std::this_thread::sleep_for(std::chrono::microseconds(20));
+    }
+  }
   VkResult res = VK_SUCCESS;
   for(uint32_t i = 0; i < n; i++)
   {
@@ -341,8 +359,6 @@ VKAPI_ATTR VkResult VKAPI_CALL vkWaitForFences(VkDevice, uint32_t n, const VkFen
       return VK_TIMEOUT;
     if(f->status != VK_SUCCESS)
       res = f->status;
-    if(!waitAll)
-      return res;
   }
   return res;
 }
@@ -699,20 +715,39 @@ VKAPI_ATTR VkResult VKAPI_CALL vkCreateGraphicsPipelines(VkDevice, VkPipelineCac
     memset(p->vattrs, 0, sizeof(p->vattrs));
     uint32_t strides[16] = {0};
     const VkPipelineVertexInputStateCreateInfo *vi = ci.pVertexInputState;
+    // Unsupported state is an error here, not a silent substitute: vertex buffers bind to slots
+    // 0..3 (the reference's GPUState has vbs[4], gpu.h:3-23) and colour blending runs the ADD
+    // op only (the reference leaves every other op's result undefined, rasterizer.cpp:657-669).
+    const char *unsupported = nullptr;
     if(vi)
     {
       for(uint32_t b = 0; b < vi->vertexBindingDescriptionCount; b++)
-        if(vi->pVertexBindingDescriptions[b].binding < 16)
-          strides[vi->pVertexBindingDescriptions[b].binding] = vi->pVertexBindingDescriptions[b].stride;
+        if(vi->pVertexBindingDescriptions[b].binding >= 4)
+          unsupported = "vertex binding beyond slot 3";
+      for(uint32_t a = 0; a < vi->vertexAttributeDescriptionCount; a++)
+        if(vi->pVertexAttributeDescriptions[a].binding >= 4 || vi->pVertexAttributeDescriptions[a].location >= 16)
+          unsupported = "vertex attribute on a binding beyond slot 3 or a location beyond 15";
+    }
+    if(ci.pColorBlendState && ci.pColorBlendState->attachmentCount > 0 &&
+       ci.pColorBlendState->pAttachments[0].blendEnable && ci.pColorBlendState->pAttachments[0].colorBlendOp != VK_BLEND_OP_ADD)
+      unsupported = "colour blend op other than ADD";
+    if(unsupported)
+    {
+      SetError(std::string("vkCreateGraphicsPipelines: unsupported: ") + unsupported);
+      delete p;
+      return VK_ERROR_FEATURE_NOT_PRESENT;
+    }
+    if(vi)
+    {
+      for(uint32_t b = 0; b < vi->vertexBindingDescriptionCount; b++)
+        strides[vi->pVertexBindingDescriptions[b].binding] = vi->pVertexBindingDescriptions[b].stride;
       for(uint32_t a = 0; a < vi->vertexAttributeDescriptionCount; a++)
       {
         const VkVertexInputAttributeDescription &ad = vi->pVertexAttributeDescriptions[a];
-        if(ad.location >= 16)
-          continue;
         p->vattrs[ad.location].vb = ad.binding;
         p->vattrs[ad.location].format = ad.format;
         p->vattrs[ad.location].offset = ad.offset;
-        p->vattrs[ad.location].stride = ad.binding < 16 ? strides[ad.binding] : 0;
+        p->vattrs[ad.location].stride = strides[ad.binding];
       }
     }
     p->topology = ci.pInputAssemblyState ? ci.pInputAssemblyState->topology : VK_PRIMITIVE_TOPOLOGY_TRIANGLE_LIST;
@@ -1019,13 +1054,16 @@ VKAPI_ATTR VkResult VKAPI_CALL vkGetPhysicalDeviceSurfacePresentModesKHR(VkPhysi
   return VK_SUCCESS;
 }
 
+VKAPI_ATTR void VKAPI_CALL vkDestroySwapchainKHR(VkDevice device, VkSwapchainKHR s, const VkAllocationCallbacks *);
+
 VKAPI_ATTR VkResult VKAPI_CALL vkCreateSwapchainKHR(VkDevice device, const VkSwapchainCreateInfoKHR *info, const VkAllocationCallbacks *,
                                                     VkSwapchainKHR *p)
 {
   VkSwapchainKHR s = new VkSwapchainKHR_T;
   s->extent = info->imageExtent;
   s->format = info->imageFormat;
-  uint32_t n = std::max(1u, std::min(info->minImageCount, 2u));
+  // minImageCount back buffers (reference wsi.cpp:96), host-visible so a present can be read back
+  uint32_t n = std::max(1u, std::min(info->minImageCount, 16u));
   for(uint32_t i = 0; i < n; i++)
   {
     VkImageCreateInfo ici;
@@ -1042,7 +1080,11 @@ VKAPI_ATTR VkResult VKAPI_CALL vkCreateSwapchainKHR(VkDevice device, const VkSwa
     VkDeviceMemory mem;
     VkResult r = vkAllocateMemory(device, &mai, nullptr, &mem);
     if(r != VK_SUCCESS)
+    {
+      vkDestroyImage(device, img, nullptr);
+      vkDestroySwapchainKHR(device, s, nullptr);
       return r;
+    }
     vkBindImageMemory(device, img, mem, 0);
     s->images.push_back(img);
     s->mems.push_back(mem);
@@ -1079,6 +1121,9 @@ VKAPI_ATTR VkResult VKAPI_CALL vkGetSwapchainImagesKHR(VkDevice, VkSwapchainKHR 
 
 VKAPI_ATTR VkResult VKAPI_CALL vkAcquireNextImageKHR(VkDevice, VkSwapchainKHR s, uint64_t, VkSemaphore, VkFence fence, uint32_t *idx)
 {
+  // reference wsi.cpp:185-195: the next image in turn, no waiting (images are never in use by
+  // a display here)
+  s->current = (s->current + 1) % (uint32_t)s->images.size();
   *idx = s->current;
   if(fence)
   {
@@ -1089,18 +1134,33 @@ VKAPI_ATTR VkResult VKAPI_CALL vkAcquireNextImageKHR(VkDevice, VkSwapchainKHR s,
   return VK_SUCCESS;
 }
 
+// Headless present (reference wsi.cpp:201-219 blits pImageIndices[i] to a window): once the
+// queue's work is done, the presented image's host-visible memory holds the frame
+// (vh_swapchain_frame), and the queue's frame statistics end and begin as the reference's
+// EndFrameStats / BeginFrameStats do (stats.cpp:25-41; vh_get_stats 2 and 3).
 VKAPI_ATTR VkResult VKAPI_CALL vkQueuePresentKHR(VkQueue queue, const VkPresentInfoKHR *info)
 {
   queue->q->WaitIdle();
+  VkResult res = VK_SUCCESS;
   for(uint32_t i = 0; i < info->swapchainCount; i++)
   {
     VkSwapchainKHR s = info->pSwapchains[i];
-    s->presents++;
-    s->current = (s->current + 1) % (uint32_t)s->images.size();
+    VkResult r = VK_SUCCESS;
+    if(!info->pImageIndices || info->pImageIndices[i] >= s->images.size())
+      r = VK_ERROR_OUT_OF_DATE_KHR;
+    else
+    {
+      s->presented = info->pImageIndices[i];
+      s->presents++;
+    }
     if(info->pResults)
-      info->pResults[i] = VK_SUCCESS;
+      info->pResults[i] = r;
+    if(r != VK_SUCCESS)
+      res = r;
   }
-  return VK_SUCCESS;
+  queue->q->presented_stats = queue->q->frame_stats;
+  queue->q->frame_stats = ExecStats();
+  return res;
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -1316,7 +1376,12 @@ int vh_get_stats(VkQueue queue, int which, VhStats *out)
 {
   if(!queue || !out)
     return -1;
-  const ExecStats &s = which == 0 ? queue->q->last_stats : queue->q->accum_stats;
+  if(which < 0 || which > 3)
+    return -1;
+  const ExecStats &s = which == 0 ? queue->q->last_stats
+                       : which == 1 ? queue->q->accum_stats
+                       : which == 2 ? queue->q->presented_stats
+                                    : queue->q->frame_stats;
   out->gpu_ms = s.gpu_ms_total;
   out->raster_ms = s.raster_ms;
   out->setup_ms = s.setup_ms;
@@ -1339,6 +1404,19 @@ int vh_get_stats(VkQueue queue, int which, VhStats *out)
   out->chunk = s.last_cfg.chunk;
   out->persistent = s.last_cfg.persistent;
   out->tuning = s.last_tuning ? 1u : 0u;
+  return 0;
+}
+
+int vh_swapchain_frame(VkSwapchainKHR swapchain, uint32_t *image_index, uint64_t *presents, const void **pixels)
+{
+  if(!swapchain || swapchain->images.empty())
+    return -1;
+  if(image_index)
+    *image_index = swapchain->presented;
+  if(presents)
+    *presents = swapchain->presents;
+  if(pixels)
+    *pixels = swapchain->presents ? swapchain->mems[swapchain->presented]->host : nullptr;
   return 0;
 }
 

@@ -244,7 +244,8 @@ struct VkSwapchainKHR_T
   VkFormat format = VK_FORMAT_B8G8R8A8_UNORM;
   std::vector<VkImage> images;
   std::vector<VkDeviceMemory> mems;
-  uint32_t current = 0;
+  uint32_t current = 0;       // last acquired image (reference wsi.cpp:185-195 advances, then returns it)
+  uint32_t presented = 0;     // image shown by the last vkQueuePresentKHR
   uint64_t presents = 0;
 };
 
@@ -360,6 +361,9 @@ struct ExecStats
   VaCounters ctr{};
 };
 
+// acc += st: counters and times summed, launch shape and tuning flag of the latest
+void AccumulateStats(ExecStats &acc, const ExecStats &st);
+
 enum class ExecMode
 {
   Plain = 0,    // per draw: setup, bin, raster, host sync (the reference's blocking DrawTriangles)
@@ -402,6 +406,8 @@ public:
   VkResult ExecuteBatch(std::vector<Submission *> &subs, ExecStats &st);
   ExecStats last_stats;
   ExecStats accum_stats;
+  ExecStats frame_stats;      // submissions since the last vkQueuePresentKHR
+  ExecStats presented_stats;  // submissions of the frame the last present ended
   ExecMode mode = ExecMode::Batched;
   TileConfig tiles;
   bool async = false;

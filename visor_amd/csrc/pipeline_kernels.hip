@@ -20,6 +20,10 @@
 #define VA_TILE_MAX_SHIFT 5
 #endif
 
+// int32 difference modulo 2^32 (window coordinates of far-off vertices reach the wrapping range;
+// the reference's x64 arithmetic wraps there)
+VA_INLINE u32 va_wsub(i32 a, i32 b) { return (u32)a - (u32)b; }
+
 VA_INLINE u32 va_find_draw(const VaDraw *__restrict__ draws, u32 ndraws, u32 g)
 {
   u32 lo = 0, hi = ndraws;
@@ -210,18 +214,18 @@ extern "C" __global__ __launch_bounds__(VA_SETUP_PRIMS) void va_setup(
     n_in = 1;
     u32 cnt = 0;
     // double_triarea (rasterizer.cpp:272-275), wrapping int32
-    const i32 area2 = (i32)((u32)(X[1] - X[0]) * (u32)(Y[2] - Y[0]) - (u32)(Y[1] - Y[0]) * (u32)(X[2] - X[0]));
+    const i32 area2 = (i32)(va_wsub(X[1], X[0]) * va_wsub(Y[2], Y[0]) - va_wsub(Y[1], Y[0]) * va_wsub(X[2], X[0]));
     if(area2 != 0)
     {
       // facing and cull (rasterizer.cpp:401-424)
       i32 flipped = area2, barymul = 1;
       bool culled = false;
-      if(d.front_face == 1u) { barymul = -1; flipped = -flipped; }
+      if(d.front_face == 1u) { barymul = -1; flipped = (i32)(0u - (u32)flipped); }
       if(flipped > 0 && (d.cull_mode & 1u)) culled = true;
       if(!culled && flipped < 0)
       {
         if(d.cull_mode & 2u) culled = true;
-        else { barymul = -barymul; flipped = -flipped; }
+        else { barymul = -barymul; flipped = (i32)(0u - (u32)flipped); }
       }
       if(!culled)
       {
@@ -246,10 +250,10 @@ extern "C" __global__ __launch_bounds__(VA_SETUP_PRIMS) void va_setup(
           r[VA_R_X0] = (u32)X[0];
           r[VA_R_Y0] = (u32)Y[0];
           // barymul folded into the edge coefficients: exact in wrapping arithmetic
-          r[VA_R_ABX] = (u32)(X[1] - X[0]) * bm;
-          r[VA_R_ABY] = (u32)(Y[1] - Y[0]) * bm;
-          r[VA_R_ACX] = (u32)(X[2] - X[0]) * bm;
-          r[VA_R_ACY] = (u32)(Y[2] - Y[0]) * bm;
+          r[VA_R_ABX] = va_wsub(X[1], X[0]) * bm;
+          r[VA_R_ABY] = va_wsub(Y[1], Y[0]) * bm;
+          r[VA_R_ACX] = va_wsub(X[2], X[0]) * bm;
+          r[VA_R_ACY] = va_wsub(Y[2], Y[0]) * bm;
           r[VA_R_AREA] = (u32)flipped;
           r[VA_R_INVAREA] = va_f2u(1.0f / (float)flipped);
           const float iw0 = 1.0f / pos[0][3], iw1 = 1.0f / pos[1][3], iw2 = 1.0f / pos[2][3];
@@ -395,8 +399,8 @@ VA_INLINE float va_half_val(u32 b) { return (float)__builtin_bit_cast(_Float16, 
 // (rasterizer.cpp:282-310, 551-588). Returns coverage; writes the pre-perspective depth.
 VA_INLINE bool va_bary(const u32 *__restrict__ r, i32 x, i32 y, float b[4], float *depth)
 {
-  const i32 PAx = (i32)r[VA_R_X0] - x;
-  const i32 PAy = (i32)r[VA_R_Y0] - y;
+  const i32 PAx = (i32)(r[VA_R_X0] - (u32)x);
+  const i32 PAy = (i32)(r[VA_R_Y0] - (u32)y);
   const u32 ux = r[VA_R_ACX] * (u32)PAy - r[VA_R_ACY] * (u32)PAx;
   const u32 uy = (u32)PAx * r[VA_R_ABY] - (u32)PAy * r[VA_R_ABX];
   const i32 b0 = (i32)(r[VA_R_AREA] - (ux + uy)), b1 = (i32)ux, b2 = (i32)uy;
@@ -537,8 +541,8 @@ VA_INLINE u32 va_wave_exclusive_sum(u32 v, u32 lane)
 // Coverage only (the integer part of va_bary).
 VA_INLINE bool va_covered(const u32 *__restrict__ r, i32 x, i32 y)
 {
-  const i32 PAx = (i32)r[VA_R_X0] - x;
-  const i32 PAy = (i32)r[VA_R_Y0] - y;
+  const i32 PAx = (i32)(r[VA_R_X0] - (u32)x);
+  const i32 PAy = (i32)(r[VA_R_Y0] - (u32)y);
   const u32 ux = r[VA_R_ACX] * (u32)PAy - r[VA_R_ACY] * (u32)PAx;
   const u32 uy = (u32)PAx * r[VA_R_ABY] - (u32)PAy * r[VA_R_ABX];
   return (i32)(r[VA_R_AREA] - (ux + uy)) >= 0 && (i32)ux >= 0 && (i32)uy >= 0;
@@ -557,8 +561,8 @@ VA_INLINE bool va_covered(const u32 *__restrict__ r, i32 x, i32 y)
 // Coverage and interpolated depth only (the first half of va_bary, same operations).
 VA_INLINE bool va_cov_depth(const u32 *__restrict__ r, i32 x, i32 y, float *depth)
 {
-  const i32 PAx = (i32)r[VA_R_X0] - x;
-  const i32 PAy = (i32)r[VA_R_Y0] - y;
+  const i32 PAx = (i32)(r[VA_R_X0] - (u32)x);
+  const i32 PAy = (i32)(r[VA_R_Y0] - (u32)y);
   const u32 ux = r[VA_R_ACX] * (u32)PAy - r[VA_R_ACY] * (u32)PAx;
   const u32 uy = (u32)PAx * r[VA_R_ABY] - (u32)PAy * r[VA_R_ABX];
   const i32 b0 = (i32)(r[VA_R_AREA] - (ux + uy)), b1 = (i32)ux, b2 = (i32)uy;
@@ -751,7 +755,7 @@ VA_INLINE void va_raster_tile(const VaDraw *__restrict__ draws, const VaFrame *_
           for(i32 y = y0; y < y1; y++)
             for(i32 x = x0; x < x1; x++)
             {
-              const i32 PAx = X0 - x, PAy = Y0 - y;
+              const i32 PAx = (i32)((u32)X0 - (u32)x), PAy = (i32)((u32)Y0 - (u32)y);
               const u32 ux = ACX * (u32)PAy - ACY * (u32)PAx;
               const u32 uy = (u32)PAx * ABY - (u32)PAy * ABX;
               if((i32)(AR - (ux + uy)) >= 0 && (i32)ux >= 0 && (i32)uy >= 0)

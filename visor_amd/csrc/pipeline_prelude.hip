@@ -15,7 +15,15 @@ typedef unsigned char u8;
 
 VA_INLINE float va_u2f(u32 u) { return __uint_as_float(u); }
 VA_INLINE u32 va_f2u(float f) { return __float_as_uint(f); }
-VA_INLINE i32 va_f2i(float f) { return (i32)f; }
+// float -> int32 as the reference's x64 build converts (cvttss2si): truncation inside
+// [-2^31, 2^31), 0x80000000 for NaN and everything outside (gfx950's conversion saturates instead)
+VA_INLINE i32 va_f2i(float f) { return (f >= -2147483648.0f && f < 2147483648.0f) ? (i32)f : (i32)0x80000000u; }
+// float -> uint32 as x64 code converts it (through a 64-bit signed conversion, low 32 bits kept;
+// 0 for NaN and everything outside [-2^63, 2^63))
+VA_INLINE u32 va_f2u32(float f)
+{
+  return (f >= -9223372036854775808.0f && f < 9223372036854775808.0f) ? (u32)(long long)f : 0u;
+}
 
 // byte / 255.0f, correctly rounded, without a division: 1/255 split into hi + lo (hi = RN(1/255),
 // lo = RN(1/255 - hi)), then one product and one FMA, b * hi + RN(b * lo), rounded once. Equal to

@@ -1394,7 +1394,7 @@ bool Translator::EmitInst(Frame &f, const Inst &in, uint32_t cur_label)
           case OpConvertSToF: e = "F((float)(i32)" + a.c[k] + ")"; break;
           case OpConvertUToF: e = "F((float)" + a.c[k] + ")"; break;
           case OpConvertFToS: e = "(u32)va_f2i(cv(" + a.c[k] + "))"; break;
-          case OpConvertFToU: e = "(u32)cv(" + a.c[k] + ")"; break;
+          case OpConvertFToU: e = "va_f2u32(cv(" + a.c[k] + "))"; break;
           default:
             if(src == dst) e = a.c[k];
             else if(src == LF) e = "va_f2u(cv(" + a.c[k] + "))";

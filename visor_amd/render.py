@@ -90,13 +90,16 @@ class Context:
         self.icd.vkGetImageMemoryRequirements(self.device, img, C.byref(req))
         mem = self.alloc(req.size, host_visible)
         self.icd.check(self.icd.vkBindImageMemory(self.device, img, mem, 0), "vkBindImageMemory")
+        return img, mem, self.view(img, fmt, mips, layers, cube), req.size
+
+    def view(self, img, fmt, mips=1, layers=1, cube=False):
         view = Handle()
         view_type = vk.VIEW_TYPE_CUBE if cube else (vk.VIEW_TYPE_2D_ARRAY if layers > 1 else vk.VIEW_TYPE_2D)
         vci = vk.ImageViewCreateInfo(image=img, viewType=view_type, format=fmt,
                                      subresourceRange=vk.ImageSubresourceRange(aspectMask=1, levelCount=mips,
                                                                                layerCount=layers))
         self.icd.check(self.icd.vkCreateImageView(self.device, C.byref(vci), None, C.byref(view)), "vkCreateImageView")
-        return img, mem, view, req.size
+        return view
 
     def texture(self, t):
         """Uploads a scenes.Texture (all levels and layers) with buffer-to-image copies."""
@@ -177,7 +180,9 @@ class Context:
 class SceneRenderer:
     """Builds the Vulkan objects of a Scene once and records/replays its render pass."""
 
-    def __init__(self, ctx, scene, host_visible_targets=True):
+    def __init__(self, ctx, scene, host_visible_targets=True, target=None):
+        """target: (image, view) of an existing colour image of the scene's size and format (a
+        swapchain image, say) to render into instead of an image of its own."""
         self.ctx, self.scene = ctx, scene
         i, dev = ctx.icd, ctx.device
         s = scene
@@ -187,8 +192,12 @@ class SceneRenderer:
             self.ib, self.ibm = ctx.upload_buffer(s.index_data)
         self.textures = [ctx.texture(t) for t in s.textures]
         # render targets
-        self.color, self.cmem, self.cview, self.csize = ctx.image(s.width, s.height, s.color_format,
-                                                                 host_visible=host_visible_targets)
+        if target is not None:
+            self.color, self.cview = target
+            self.cmem = None
+        else:
+            self.color, self.cmem, self.cview, self.csize = ctx.image(s.width, s.height, s.color_format,
+                                                                     host_visible=host_visible_targets)
         self.has_depth = s.clear_depth is not None
         views = [self.cview]
         if self.has_depth:
@@ -217,12 +226,14 @@ class SceneRenderer:
                                        width=s.width, height=s.height, layers=1)
         self.fb = Handle()
         i.check(i.vkCreateFramebuffer(dev, C.byref(fci), None, C.byref(self.fb)), "vkCreateFramebuffer")
-        # descriptors: set 0 binding 0 = combined image sampler
-        binding = vk.DescriptorSetLayoutBinding(binding=0, descriptorType=vk.DESC_COMBINED_IMAGE_SAMPLER, descriptorCount=1,
-                                                stageFlags=vk.STAGE_FRAGMENT)
+        # descriptors: set 0 binding 0 = combined image sampler, binding 1 = uniform buffer
+        binding = (vk.DescriptorSetLayoutBinding * 2)(
+            vk.DescriptorSetLayoutBinding(binding=0, descriptorType=vk.DESC_COMBINED_IMAGE_SAMPLER, descriptorCount=1,
+                                          stageFlags=vk.STAGE_FRAGMENT),
+            vk.DescriptorSetLayoutBinding(binding=shaders.UBO_BINDING, descriptorType=vk.DESC_UNIFORM_BUFFER,
+                                          descriptorCount=1, stageFlags=vk.STAGE_VERTEX | vk.STAGE_FRAGMENT))
         self.dsl = Handle()
-        i.check(i.vkCreateDescriptorSetLayout(dev, C.byref(vk.DescriptorSetLayoutCreateInfo(bindingCount=1,
-                                                                                           pBindings=C.pointer(binding))),
+        i.check(i.vkCreateDescriptorSetLayout(dev, C.byref(vk.DescriptorSetLayoutCreateInfo(bindingCount=2, pBindings=binding)),
                                               None, C.byref(self.dsl)), "vkCreateDescriptorSetLayout")
         pcr = vk.PushConstantRange(stageFlags=vk.STAGE_VERTEX | vk.STAGE_FRAGMENT, offset=0, size=128)
         dsl_arr = (Handle * 1)(self.dsl)
@@ -231,19 +242,34 @@ class SceneRenderer:
                                                                                   pushConstantRangeCount=1,
                                                                                   pPushConstantRanges=C.pointer(pcr))),
                                          None, C.byref(self.layout)), "vkCreatePipelineLayout")
+        # uniform buffers, and one set per (texture, uniform buffer) pair that a draw binds with a
+        # uniform buffer; self.sets[t] is the set of texture t alone
+        self.uniform_bufs = [ctx.upload_buffer(u) for u in s.uniforms]
+        ucombos = sorted({(d.texture, d.uniform) for d in s.draws if d.uniform is not None},
+                         key=lambda k: (-1 if k[0] is None else k[0], k[1]))
         self.dpool = Handle()
-        i.check(i.vkCreateDescriptorPool(dev, C.byref(vk.DescriptorPoolCreateInfo(maxSets=max(1, len(self.textures)))), None,
-                                         C.byref(self.dpool)), "vkCreateDescriptorPool")
-        self.sets = []
-        for (img, mem, view, samp) in self.textures:
+        i.check(i.vkCreateDescriptorPool(dev, C.byref(vk.DescriptorPoolCreateInfo(
+            maxSets=max(1, len(self.textures) + len(ucombos)))), None, C.byref(self.dpool)), "vkCreateDescriptorPool")
+
+        def new_set(tex, ubo):
             ds = Handle()
             dai = vk.DescriptorSetAllocateInfo(descriptorPool=self.dpool, descriptorSetCount=1, pSetLayouts=dsl_arr)
             i.check(i.vkAllocateDescriptorSets(dev, C.byref(dai), C.byref(ds)), "vkAllocateDescriptorSets")
-            dii = vk.DescriptorImageInfo(sampler=samp, imageView=view, imageLayout=5)
-            w = vk.WriteDescriptorSet(dstSet=ds, dstBinding=0, descriptorCount=1, descriptorType=vk.DESC_COMBINED_IMAGE_SAMPLER,
-                                      pImageInfo=C.pointer(dii))
-            i.vkUpdateDescriptorSets(dev, 1, C.byref(w), 0, None)
-            self.sets.append(ds)
+            if tex is not None:
+                img, mem, view, samp = self.textures[tex]
+                dii = vk.DescriptorImageInfo(sampler=samp, imageView=view, imageLayout=5)
+                w = vk.WriteDescriptorSet(dstSet=ds, dstBinding=0, descriptorCount=1,
+                                          descriptorType=vk.DESC_COMBINED_IMAGE_SAMPLER, pImageInfo=C.pointer(dii))
+                i.vkUpdateDescriptorSets(dev, 1, C.byref(w), 0, None)
+            if ubo is not None:
+                dbi = vk.DescriptorBufferInfo(buffer=self.uniform_bufs[ubo][0], offset=0, range=len(s.uniforms[ubo]))
+                w = vk.WriteDescriptorSet(dstSet=ds, dstBinding=shaders.UBO_BINDING, descriptorCount=1,
+                                          descriptorType=vk.DESC_UNIFORM_BUFFER, pBufferInfo=C.pointer(dbi))
+                i.vkUpdateDescriptorSets(dev, 1, C.byref(w), 0, None)
+            return ds
+
+        self.sets = [new_set(t, None) for t in range(len(self.textures))]
+        self.usets = {k: new_set(*k) for k in ucombos}
         # pipelines
         self.modules = {}
         self.pipelines = {}
@@ -318,14 +344,15 @@ class SceneRenderer:
             if d.pipeline != cur_pipe:
                 i.vkCmdBindPipeline(cb, 0, self.pipelines[d.pipeline])
                 cur_pipe = d.pipeline
-            if d.texture is not None and d.texture != cur_set:
-                sets = (Handle * 1)(self.sets[d.texture])
+            key = (d.texture, d.uniform)
+            if key != (None, None) and key != cur_set:
+                sets = (Handle * 1)(self.usets[key] if d.uniform is not None else self.sets[d.texture])
                 i.vkCmdBindDescriptorSets(cb, 0, self.layout, 0, 1, sets, 0, None)
-                cur_set = d.texture
+                cur_set = key
             pc = C.create_string_buffer(d.push, 128)
             i.vkCmdPushConstants(cb, self.layout, vk.STAGE_VERTEX | vk.STAGE_FRAGMENT, 0, 128, pc)
             if d.indexed:
-                i.vkCmdDrawIndexed(cb, d.count, 1, d.first, 0, 0)
+                i.vkCmdDrawIndexed(cb, d.count, 1, d.first, d.vertex_offset, 0)
             else:
                 i.vkCmdDraw(cb, d.count, 1, d.first, 0)
         i.vkCmdEndRenderPass(cb)

@@ -90,6 +90,8 @@ class Draw:
     indexed: bool = False
     push: bytes = b"\0" * 128
     texture: Optional[int] = None
+    vertex_offset: int = 0   # vkCmdDrawIndexed's vertexOffset (recorded; the reference never applies it)
+    uniform: Optional[int] = None  # index into Scene.uniforms, bound at set 0 binding 1
 
 
 @dataclass
@@ -108,6 +110,7 @@ class Scene:
     color_format: int = FMT["B8G8R8A8_UNORM"]
     clear_color: tuple = (0.0, 0.0, 0.0, 0.0)
     clear_depth: Optional[float] = 1.0
+    uniforms: List[bytes] = field(default_factory=list)  # uniform-buffer contents the draws bind
 
     @property
     def triangles(self):

@@ -193,10 +193,65 @@ def fs_multitap8():
     return m.to_bytes()
 
 
+# Uniform block of the UBO-fed programs: set 0, binding 1 (std140 offsets)
+#     0   mat4 mvp
+#     64  vec4 tint
+#     80  vec4 bias
+UBO_BINDING = 1
+
+
+def _ubo_block(m):
+    return m.uniform_block([m.mat(4, 4), m.vec(4), m.vec(4)], [0, 64, 80], key="UBO", set_=0, binding=UBO_BINDING)
+
+
+def vs_ubo_pos4_col():
+    """gl_Position = ubo.mvp * pos; out0 = col.  (oracle RR_VS_UBO_POS4_COL)"""
+    m = Module()
+    v4, mat4 = m.vec(4), m.mat(4, 4)
+    _, ubo = _ubo_block(m)
+    pos = m.input(v4, location=0)
+    col = m.input(v4, location=1)
+    out_col = m.output(v4, location=0)
+    pv = _per_vertex_position(m)
+    m.begin_main()
+    i0 = m.const_i(0)
+    mvp = m.load(mat4, m.access("Uniform", mat4, ubo, i0))
+    m.store(m.access("Output", v4, pv, i0), m.op("MatrixTimesVector", v4, mvp, m.load(v4, pos)))
+    m.store(out_col, m.load(v4, col))
+    m.end_main("Vertex", "main", [pos, col, out_col, pv])
+    return m.to_bytes()
+
+
+def fs_ubo_tint():
+    """out = in0 * ubo.tint + ubo.bias (component-wise).  (oracle RR_FS_UBO_TINT)"""
+    m = Module()
+    v4 = m.vec(4)
+    _, ubo = _ubo_block(m)
+    c = m.input(v4, location=0)
+    o = m.output(v4, location=0)
+    m.begin_main()
+    tint = m.load(v4, m.access("Uniform", v4, ubo, m.const_i(1)))
+    bias = m.load(v4, m.access("Uniform", v4, ubo, m.const_i(2)))
+    m.store(o, m.op("FAdd", v4, m.op("FMul", v4, m.load(v4, c), tint), bias))
+    m.end_main("Fragment", "main", [c, o], frag=True)
+    return m.to_bytes()
+
+
+def ubo_block_bytes(mvp, tint, bias):
+    """The uniform block's bytes: column-major mvp (given row-major), tint, bias."""
+    import numpy as np
+    b = bytearray(96)
+    b[0:64] = np.asarray(mvp, dtype=np.float32).T.tobytes()
+    b[64:80] = np.asarray(tint, dtype=np.float32).tobytes()
+    b[80:96] = np.asarray(bias, dtype=np.float32).tobytes()
+    return bytes(b)
+
+
 # oracle program ids (oracle/refrast.h)
 ORACLE_ID = {
     "vs_mvp_pos4_col": 1, "vs_mvp_normal_uv": 2, "fs_color": 101, "fs_lambert_tex": 102,
     "fs_lambert_tex_a": 103, "fs_multitap8": 104, "fs_tex_only": 105, "fs_cube_env": 106,
+    "vs_ubo_pos4_col": 3, "fs_ubo_tint": 107,
 }
 
 PROGRAMS = {
@@ -208,4 +263,6 @@ PROGRAMS = {
     "fs_multitap8": fs_multitap8,
     "fs_tex_only": fs_tex_only,
     "fs_cube_env": fs_cube_env,
+    "vs_ubo_pos4_col": vs_ubo_pos4_col,
+    "fs_ubo_tint": fs_ubo_tint,
 }

@@ -180,6 +180,20 @@ class Module:
                 self.member_decorate(st, m, "MatrixStride", matrix_stride)
         return st, self.variable("PushConstant", st)
 
+    def uniform_block(self, members, offsets, key, set_, binding, matrix_stride=16):
+        """A uniform buffer block at (set_, binding); members are type ids, offsets byte offsets."""
+        st = self.struct(members, key)
+        self.decorate(st, "Block")
+        for m, (t, off) in enumerate(zip(members, offsets)):
+            self.member_decorate(st, m, "Offset", off)
+            if ("mat" in str(self._type_key(t))):
+                self.member_decorate(st, m, "ColMajor")
+                self.member_decorate(st, m, "MatrixStride", matrix_stride)
+        v = self.variable("Uniform", st)
+        self.decorate(v, "DescriptorSet", set_)
+        self.decorate(v, "Binding", binding)
+        return st, v
+
     def _type_key(self, t):
         for k, v in self._type_cache.items():
             if v == t:

@@ -27,7 +27,8 @@ ST = dict(APPLICATION_INFO=0, INSTANCE_CREATE_INFO=1, DEVICE_QUEUE_CREATE_INFO=2
           PIPELINE_LAYOUT_CREATE_INFO=30, SAMPLER_CREATE_INFO=31, DESCRIPTOR_SET_LAYOUT_CREATE_INFO=32,
           DESCRIPTOR_POOL_CREATE_INFO=33, DESCRIPTOR_SET_ALLOCATE_INFO=34, WRITE_DESCRIPTOR_SET=35,
           FRAMEBUFFER_CREATE_INFO=37, RENDER_PASS_CREATE_INFO=38, COMMAND_POOL_CREATE_INFO=39,
-          COMMAND_BUFFER_ALLOCATE_INFO=40, COMMAND_BUFFER_BEGIN_INFO=42, RENDER_PASS_BEGIN_INFO=43)
+          COMMAND_BUFFER_ALLOCATE_INFO=40, COMMAND_BUFFER_BEGIN_INFO=42, RENDER_PASS_BEGIN_INFO=43,
+          SWAPCHAIN_CREATE_INFO_KHR=1000001000, PRESENT_INFO_KHR=1000001001)
 
 FMT = dict(R8_UNORM=9, R8G8B8A8_UNORM=37, B8G8R8A8_UNORM=44, R16G16B16A16_SFLOAT=97, R32_SFLOAT=100,
            R32G32_SFLOAT=103, R32G32B32_SFLOAT=106, R32G32B32A32_SFLOAT=109, D32_SFLOAT=126,
@@ -92,6 +93,15 @@ DeviceCreateInfo = _st("DeviceCreateInfo", [("sType", u32), ("pNext", vp), ("fla
                                             ("pQueueCreateInfos", C.POINTER(DeviceQueueCreateInfo)), ("enabledLayerCount", u32),
                                             ("ppEnabledLayerNames", vp), ("enabledExtensionCount", u32),
                                             ("ppEnabledExtensionNames", vp), ("pEnabledFeatures", vp)], "DEVICE_CREATE_INFO")
+SwapchainCreateInfoKHR = _st("SwapchainCreateInfoKHR", [
+    ("sType", u32), ("pNext", vp), ("flags", u32), ("surface", Handle), ("minImageCount", u32), ("imageFormat", u32),
+    ("imageColorSpace", u32), ("imageExtent", Extent2D), ("imageArrayLayers", u32), ("imageUsage", u32),
+    ("imageSharingMode", u32), ("queueFamilyIndexCount", u32), ("pQueueFamilyIndices", vp), ("preTransform", u32),
+    ("compositeAlpha", u32), ("presentMode", u32), ("clipped", u32), ("oldSwapchain", Handle)], "SWAPCHAIN_CREATE_INFO_KHR")
+PresentInfoKHR = _st("PresentInfoKHR", [
+    ("sType", u32), ("pNext", vp), ("waitSemaphoreCount", u32), ("pWaitSemaphores", vp), ("swapchainCount", u32),
+    ("pSwapchains", C.POINTER(Handle)), ("pImageIndices", C.POINTER(u32)), ("pResults", C.POINTER(i32))],
+    "PRESENT_INFO_KHR")
 SubmitInfo = _st("SubmitInfo", [("sType", u32), ("pNext", vp), ("waitSemaphoreCount", u32), ("pWaitSemaphores", vp),
                                 ("pWaitDstStageMask", vp), ("commandBufferCount", u32), ("pCommandBuffers", C.POINTER(Handle)),
                                 ("signalSemaphoreCount", u32), ("pSignalSemaphores", vp)], "SUBMIT_INFO")
@@ -319,6 +329,11 @@ PROTOS = [
     ("vkCmdCopyBuffer", None, [Handle, Handle, Handle, u32, P(BufferCopy)]),
     ("vkCmdPipelineBarrier", None, [Handle, u32, u32, u32, u32, vp, u32, vp, u32, vp]),
     ("vkQueueSubmit", i32, [Handle, u32, P(SubmitInfo), Handle]),
+    ("vkCreateSwapchainKHR", i32, [Handle, P(SwapchainCreateInfoKHR), vp, P(Handle)]),
+    ("vkDestroySwapchainKHR", None, [Handle, Handle, vp]),
+    ("vkGetSwapchainImagesKHR", i32, [Handle, Handle, P(u32), P(Handle)]),
+    ("vkAcquireNextImageKHR", i32, [Handle, Handle, u64, Handle, Handle, P(u32)]),
+    ("vkQueuePresentKHR", i32, [Handle, P(PresentInfoKHR)]),
 ]
 
 EXT_PROTOS = [
@@ -336,6 +351,7 @@ EXT_PROTOS = [
     ("vh_set_async_submit", C.c_int, [Handle, C.c_int]),
     ("vh_get_stats", C.c_int, [Handle, C.c_int, P(VhStats)]),
     ("vh_reset_stats", C.c_int, [Handle]),
+    ("vh_swapchain_frame", C.c_int, [Handle, P(u32), P(u64), P(vp)]),
     ("vh_translate_spirv", C.c_int, [P(u32), size_t, C.c_int, C.c_char_p, size_t]),
     ("vh_build_pipeline_offline", C.c_int, [P(u32), size_t, P(u32), size_t, cp, C.c_char_p, size_t]),
 ]
