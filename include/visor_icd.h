@@ -83,6 +83,7 @@ int vh_set_async_submit(VkQueue queue, int on);
  * (submissions since the last present). A present ends and begins frame statistics as the
  * reference's does (wsi.cpp:215-217, stats.cpp:25-41). */
 int vh_get_stats(VkQueue queue, int which, VhStats *out);
+/* Clears the accumulated (1) and current-frame (3) statistics. */
 int vh_reset_stats(VkQueue queue);
 /* The headless present's output (the reference blits the presented image to a window,
  * wsi.cpp:201-213): index of the image the last vkQueuePresentKHR showed, the number of

@@ -185,6 +185,7 @@ def test_headless_swapchain_present_and_frame_stats(ctx):
     idx, presents = vk.u32(), vk.u64()
     assert ctx.icd.vh_swapchain_frame(sw, C.byref(idx), C.byref(presents), C.byref(pixels)) == 0
     assert presents.value == 0 and not pixels.value
+    ctx.icd.vh_reset_stats(ctx.queue)  # the frame begins here (earlier tests shared the queue)
     for frame in range(2 * n):
         ctx.icd.check(ctx.icd.vkAcquireNextImageKHR(ctx.device, sw, 2 ** 64 - 1, None, None, C.byref(idx)), "acquire")
         assert idx.value == (frame + 1) % n

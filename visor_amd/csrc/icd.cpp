@@ -1425,6 +1425,7 @@ int vh_reset_stats(VkQueue queue)
   if(!queue)
     return -1;
   queue->q->accum_stats = ExecStats();
+  queue->q->frame_stats = ExecStats();
   return 0;
 }
 
