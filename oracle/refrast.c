@@ -1154,6 +1154,7 @@ static void draw_setup(const rr_state *st, int num_verts, uint32_t first, int in
         b.maxy = maxy < b.miny + bs ? maxy : b.miny + bs;
         emit(ctx, &b);
       }
+    emit(ctx, NULL); /* the triangle's blocks are all queued */
   }
   if(ctr)
   {
@@ -1180,7 +1181,11 @@ static void fifo_push(fifo *q, const tri_work *w)
   q->items[q->tail++] = *w;
 }
 
-static void emit_serial(void *ctx, const tri_work *w) { fifo_push((fifo *)ctx, w); }
+static void emit_serial(void *ctx, const tri_work *w)
+{
+  if(w)
+    fifo_push((fifo *)ctx, w);
+}
 
 void rr_draw(const rr_state *st, int num_verts, uint32_t first, int indexed, rr_counters *ctr)
 {
@@ -1201,6 +1206,14 @@ void rr_draw(const rr_state *st, int num_verts, uint32_t first, int indexed, rr_
 /* ---------------------------------------------------------------------------------------- */
 /* Reference-structured threaded mode (timing only): reference rasterizer.cpp:36-98,454-515.  */
 
+/* Counters: one slot per thread (workers 0..n-1, the submitter n), each on its own cache line,
+ * summed by the submitter once the draw's blocks are done (the reference's counters are
+ * compiled-out microprofile counters, so it takes no lock for them either). */
+typedef struct ctr_slot {
+  rr_counters c;
+  char pad[64];
+} ctr_slot;
+
 typedef struct pool {
   pthread_t threads[64];
   int nthreads;
@@ -1209,28 +1222,15 @@ typedef struct pool {
   int kill;
   fifo q;
   atomic_int pending;
-  rr_counters ctr;
-  pthread_mutex_t ctr_mutex;
+  ctr_slot slots[65];
 } pool;
 
 static pool g_pool;
 static int g_pool_started;
 
-static void pool_process(const tri_work *w)
-{
-  rr_counters local = {0};
-  process_block(w, &local);
-  pthread_mutex_lock(&g_pool.ctr_mutex);
-  g_pool.ctr.blocks += local.blocks;
-  g_pool.ctr.pixels_tested += local.pixels_tested;
-  g_pool.ctr.pixels_written += local.pixels_written;
-  g_pool.ctr.depth_passed += local.depth_passed;
-  pthread_mutex_unlock(&g_pool.ctr_mutex);
-}
-
 static void *raster_loop(void *arg)
 {
-  (void)arg;
+  rr_counters *mine = &g_pool.slots[(int)(intptr_t)arg].c;
   tc_init();
   for(;;)
   {
@@ -1245,7 +1245,7 @@ static void *raster_loop(void *arg)
     }
     w = g_pool.q.items[g_pool.q.head++];
     pthread_mutex_unlock(&g_pool.mutex);
-    pool_process(&w);
+    process_block(&w, mine);
     atomic_fetch_sub(&g_pool.pending, 1);
   }
 }
@@ -1272,14 +1272,20 @@ static void pool_start(int n)
   pool_stop();
   if(n > 64)
     n = 64;
-  pthread_mutex_init(&g_pool.mutex, NULL);
-  pthread_mutex_init(&g_pool.ctr_mutex, NULL);
+  /* the reference's std::mutex is an MSVC SRW lock, which spins briefly before it sleeps;
+   * glibc's adaptive mutex behaves alike */
+  pthread_mutexattr_t ma;
+  pthread_mutexattr_init(&ma);
+  pthread_mutexattr_settype(&ma, PTHREAD_MUTEX_ADAPTIVE_NP);
+  pthread_mutex_init(&g_pool.mutex, &ma);
+  pthread_mutexattr_destroy(&ma);
   pthread_cond_init(&g_pool.wake, NULL);
   g_pool.kill = 0;
   g_pool.nthreads = n;
   atomic_store(&g_pool.pending, 0);
+  memset(g_pool.slots, 0, sizeof(g_pool.slots));
   for(int i = 0; i < n; i++)
-    pthread_create(&g_pool.threads[i], NULL, raster_loop, NULL);
+    pthread_create(&g_pool.threads[i], NULL, raster_loop, (void *)(intptr_t)i);
   g_pool_started = 1;
 }
 
@@ -1287,15 +1293,20 @@ typedef struct thr_ctx {
   int dummy;
 } thr_ctx;
 
+/* reference rasterizer.cpp:466-486: each block is queued under the lock; the workers are woken
+ * once per triangle, after its blocks */
 static void emit_threaded(void *ctx, const tri_work *w)
 {
   (void)ctx;
+  if(!w)
+  {
+    pthread_cond_broadcast(&g_pool.wake);
+    return;
+  }
   pthread_mutex_lock(&g_pool.mutex);
   fifo_push(&g_pool.q, w);
   atomic_fetch_add(&g_pool.pending, 1);
   pthread_mutex_unlock(&g_pool.mutex);
-  /* the reference notifies once per triangle after its blocks; per block is equivalent work */
-  pthread_cond_broadcast(&g_pool.wake);
 }
 
 void rr_draw_threaded(const rr_state *st, int num_verts, uint32_t first, int indexed,
@@ -1303,7 +1314,7 @@ void rr_draw_threaded(const rr_state *st, int num_verts, uint32_t first, int ind
 {
   atomic_fetch_add(&tc_generation, 1);
   pool_start(nthreads);
-  memset(&g_pool.ctr, 0, sizeof(g_pool.ctr));
+  rr_counters *mine = &g_pool.slots[g_pool.nthreads].c;
   rr_vertex *sv;
   int32_t *win;
   rr_counters setup_ctr = {0};
@@ -1322,7 +1333,7 @@ void rr_draw_threaded(const rr_state *st, int num_verts, uint32_t first, int ind
     pthread_mutex_unlock(&g_pool.mutex);
     if(have)
     {
-      pool_process(&w);
+      process_block(&w, mine);
       atomic_fetch_sub(&g_pool.pending, 1);
     }
   }
@@ -1334,10 +1345,18 @@ void rr_draw_threaded(const rr_state *st, int num_verts, uint32_t first, int ind
     ctr->draws += setup_ctr.draws;
     ctr->tris_in += setup_ctr.tris_in;
     ctr->tris_out += setup_ctr.tris_out;
-    ctr->blocks += g_pool.ctr.blocks;
-    ctr->pixels_tested += g_pool.ctr.pixels_tested;
-    ctr->pixels_written += g_pool.ctr.pixels_written;
-    ctr->depth_passed += g_pool.ctr.depth_passed;
+  }
+  for(int i = 0; i <= g_pool.nthreads; i++)
+  {
+    rr_counters *c = &g_pool.slots[i].c;
+    if(ctr)
+    {
+      ctr->blocks += c->blocks;
+      ctr->pixels_tested += c->pixels_tested;
+      ctr->pixels_written += c->pixels_written;
+      ctr->depth_passed += c->depth_passed;
+    }
+    memset(c, 0, sizeof(*c));
   }
   free(sv);
   free(win);

@@ -193,9 +193,13 @@ def test_many_draws_full_bit_exact(ctx):
 
 
 def test_micro_triangles_4k_bit_exact(ctx):
-    """3840x2160 RGBA16F target, 1M micro-triangles (a quarter of the full count), 8 trilinear
-    taps of a 2048^2 x 4 array texture, a quarter of the draws additive, against the oracle."""
-    s = scenes.micro_triangles(ntris=1_000_000)
+    """The micro-triangle workload at its full size: 3840x2160 RGBA16F target, 4M micro-triangles
+    in 50 depth layers, 8 trilinear taps of a 2048^2 x 4 array texture, a quarter of the draws
+    additive, against the oracle (about a minute of oracle time), counters included."""
+    s = scenes.micro_triangles()
+    assert s.triangles == 4_000_000
     ref = refrast.render(s)
     out = _render_gpu(ctx, s, tile=8)
     _compare(s, out, ref)
+    for k in ("pixels_written", "depth_passed", "tris_in", "tris_out", "blocks", "pixels_tested"):
+        assert out["stats"][k] == ref["counters"][k], k

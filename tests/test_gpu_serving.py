@@ -73,3 +73,41 @@ def test_admission_by_memory_budget(ctx, world):
         assert st["batches"] == len(world.clients)
     finally:
         srv.close()
+
+
+@pytest.fixture(scope="module")
+def world64(ctx):
+    """The 64-client serving workload at its own shape: targets 256^2, 512^2, 1024^2 and
+    1920x1080, textures 256^2..2048^2, 1-32 draws of 1k..500k triangles in total per request
+    (depth made host-visible so that it can be compared too)."""
+    return serving.ServingWorld(ctx, nclients=64, seed=4, host_visible_depth=True)
+
+
+def test_64_clients_mix_equals_alone_and_oracle(ctx, world64):
+    w = world64
+    assert {(c.width, c.height) for c in w.clients} == set(serving.TARGET_SIZES)
+    tris = [c.triangles for c in w.clients]
+    assert 1000 <= min(tris) and max(tris) <= 500_000 and max(tris) > 100_000
+    srv = serving.Server(ctx, slots=2, max_batch=64)
+    try:
+        # the mix: every client's request at once, so admitted requests share launches
+        tickets = [srv.submit(c.cb) for c in w.clients]
+        for t in tickets:
+            srv.wait(t)
+        st = srv.stats()
+        assert st["completed"] == len(w.clients) and st["mean_batch"] > 1.0
+        mix = [(w.read_color(c), w.read_depth(c)) for c in w.clients]
+        # alone: one request on the GPU at a time
+        for c, (col, dep) in zip(w.clients, mix):
+            srv.wait(srv.submit(c.cb))
+            assert np.array_equal(w.read_color(c), col), c.index
+            assert np.array_equal(w.read_depth(c).view(np.uint32), dep.view(np.uint32)), c.index
+    finally:
+        srv.close()
+    # the oracle, for the client with the most triangles of each target size
+    for size in serving.TARGET_SIZES:
+        c = max((c for c in w.clients if (c.width, c.height) == size), key=lambda c: c.triangles)
+        ref = refrast.render(w.scene(c))
+        col, dep = mix[c.index]
+        assert np.array_equal(col, ref["color"]), (size, c.index)
+        assert np.array_equal(dep.view(np.uint32), ref["depth"].view(np.uint32)), (size, c.index)

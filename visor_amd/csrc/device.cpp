@@ -400,17 +400,26 @@ std::shared_ptr<PipelineModule> Device::GetModule(const std::vector<std::shared_
 bool BuildModuleOffline(const std::vector<std::shared_ptr<ShaderPair>> &ps, const std::string &arch,
                         const std::string &cache_dir, std::string &err, size_t *code_size, std::string *src_out)
 {
-  uint32_t stride;
-  std::string src = ModuleSource(ps, stride, TileConfig());
-  if(src_out)
-    *src_out = src;
-  std::vector<char> code;
-  std::string key;
-  double secs;
-  if(!BuildCodeObject(src, arch, cache_dir.empty() ? DefaultCacheDir() : cache_dir, code, key, secs, err))
-    return false;
-  if(code_size)
-    *code_size = code.size();
+  // the module's LDS chunk is a compile-time parameter: build both sizes the autotuner tries
+  for(uint32_t chunk : {64u, 32u})
+  {
+    TileConfig cfg;
+    cfg.chunk = chunk;
+    uint32_t stride;
+    std::string src = ModuleSource(ps, stride, cfg);
+    std::vector<char> code;
+    std::string key;
+    double secs;
+    if(!BuildCodeObject(src, arch, cache_dir.empty() ? DefaultCacheDir() : cache_dir, code, key, secs, err))
+      return false;
+    if(chunk == TileConfig().chunk)
+    {
+      if(src_out)
+        *src_out = src;
+      if(code_size)
+        *code_size = code.size();
+    }
+  }
   return true;
 }
 }  // namespace visor

@@ -93,7 +93,7 @@ Queue::~Queue()
     worker.join();
   }
   void *bufs[] = {d_frames, d_draws, d_rec, d_count, d_offs, d_keys, d_vals, d_keys2, d_vals2, d_start, d_end, d_temp,
-                  d_blocks, d_part, d_stemp, d_einfo, d_stamps};
+                  d_blocks, d_part, d_stemp, d_einfo, d_stamps, d_masks};
   for(void *b : bufs)
     if(b)
       hipFree(b);
@@ -109,6 +109,16 @@ Queue::~Queue()
     hipEventDestroy(ev_end);
   if(stream)
     hipStreamDestroy(stream);
+}
+
+// Bytes of the raster's pass masks: per tile and chunk of its bin, one mask per pixel (4 bytes for
+// chunks of at most 32 records, else 8). Tile t's masks start at chunk slot floor(start[t] /
+// chunk) + t, which leaves every tile room for its ceil(len / chunk) chunks.
+static size_t RasterMaskBytes(const TileConfig &cfg, uint32_t pair_cap, uint32_t tiles)
+{
+  const size_t px = (size_t)64 << (2 * (cfg.tile_shift - 3));
+  const size_t elem = cfg.chunk <= 32 ? 4 : 8;
+  return elem * px * ((size_t)pair_cap / cfg.chunk + tiles + 1);
 }
 
 bool Queue::Grow(void **p, size_t &cap, size_t need)
@@ -1002,7 +1012,8 @@ VkResult Queue::RunPass(const std::vector<Op *> &passes, bool fused_clear, ExecS
   {
     if(!Grow(&d_keys, c_keys, 4ull * pair_cap) || !Grow(&d_vals, c_vals, 4ull * pair_cap) ||
        !Grow(&d_keys2, c_keys2, 4ull * pair_cap) || !Grow(&d_vals2, c_vals2, 4ull * pair_cap) ||
-       !Grow(&d_stemp, c_stemp, 4 * va_sort_temp_words(pair_cap)))
+       !Grow(&d_stemp, c_stemp, 4 * va_sort_temp_words(pair_cap)) ||
+       !Grow(&d_masks, c_masks, RasterMaskBytes(cfg, pair_cap, tile_total)))
       return VK_ERROR_OUT_OF_DEVICE_MEMORY;
     VaDraw *dd = (VaDraw *)d_draws;
     VaFrame *df = (VaFrame *)d_frames;
@@ -1075,15 +1086,17 @@ VkResult Queue::RunPass(const std::vector<Op *> &passes, bool fused_clear, ExecS
         // tile t's bin is [start[t], start[t + 1])
         uint32_t *st_ = (uint32_t *)d_start, *en = (uint32_t *)d_start + 1;
         uint32_t use_fused = fused_clear ? 1u : 0u;
-        void *args[] = {&dd,         &df,        (void *)&nframes, &rec,      &bins, &st_,    &en,
-                        &tile_total, &use_fused, &part_r,          &total_dev, &cap,  &stamp_r};
+        void *masks = d_masks;
+        void *args[] = {&dd,         &df,        (void *)&nframes, &rec,      &bins,  &st_,    &en,
+                        &tile_total, &use_fused, &part_r,          &total_dev, &cap,  &masks, &stamp_r};
         uint32_t threads = (T / 8) * (T / 8) * 64;
         if(cfg.persistent)
         {
           uint32_t *tile_counter = (uint32_t *)d_stamps + 8;
           VA_TRY(va_fill_u32_async(tile_counter, 1, 0u, stream));
-          void *pargs[] = {&dd,         &df,        (void *)&nframes, &rec,       &bins, &st_,          &en,
-                           &tile_total, &use_fused, &part_r,          &total_dev, &cap,  &tile_counter, &stamp_r};
+          void *pargs[] = {&dd,         &df,        (void *)&nframes, &rec,       &bins,  &st_,          &en,
+                           &tile_total, &use_fused, &part_r,          &total_dev, &cap,  &masks, &tile_counter,
+                           &stamp_r};
           const uint32_t grid = std::min(mod->raster_p_grid[ts - 3], std::max(tile_total, 1u));
           VA_TRY(hipModuleLaunchKernel(mod->raster_p[ts - 3], grid, 1, 1, threads, 1, 1, 0, stream, pargs, nullptr));
           return hipSuccess;
@@ -1110,8 +1123,8 @@ VkResult Queue::RunPass(const std::vector<Op *> &passes, bool fused_clear, ExecS
       key.dims[5] = cap;
       key.dims[6] = fused_clear ? 1u : 0u;
       key.dims[7] = cfg.persistent;
-      void *bufs[] = {d_frames, d_draws, d_rec, d_count, d_offs, d_keys, d_vals, d_keys2, d_vals2,
-                      d_stemp, d_temp, d_start, d_einfo, d_blocks, d_part, d_stamps, h_pinned};
+      void *bufs[] = {d_frames, d_draws, d_rec,   d_count, d_offs,   d_keys,   d_vals,  d_keys2, d_vals2,
+                      d_stemp,  d_temp,  d_start, d_einfo, d_blocks, d_part,   d_stamps, d_masks, h_pinned};
       static_assert(sizeof(bufs) <= sizeof(key.bufs), "graph key holds every buffer");
       memcpy(key.bufs, bufs, sizeof(bufs));
       for(size_t i = 0; i < graph_cache.size(); i++)

@@ -454,6 +454,8 @@ private:
   size_t c_einfo = 0;
   void *d_stamps = nullptr;  // device wall-clock stamps at stage boundaries (+ the tile counter)
   size_t c_stamps = 0;
+  void *d_masks = nullptr;  // the raster's per-chunk pass masks (visibility pass -> shading pass)
+  size_t c_masks = 0;
   uint32_t pair_cap = 0;    // (tile, primitive) pair capacity of the bin buffers
   // last uploaded per-pass tables (skipped when a submission repeats them)
   std::vector<VaFrame> up_frames;

@@ -505,6 +505,74 @@ VA_INLINE void va_wave_sync()
 #endif
 static_assert(VA_CHUNK <= 64u, "record sets are 64-bit masks");
 
+// A set of records of one chunk (bit k = record k): 32 bits when a chunk holds at most 32.
+#if VA_CHUNK <= 32
+typedef u32 va_cmask;
+VA_INLINE u32 va_cmask_ctz(u32 m) { return (u32)__builtin_ctz(m); }
+VA_INLINE u32 va_cmask_pop(u32 m) { return (u32)__builtin_popcount(m); }
+VA_INLINE u32 va_cmask_top(u32 m) { return 31u - (u32)__builtin_clz(m); }
+#else
+typedef u64 va_cmask;
+VA_INLINE u32 va_cmask_ctz(u64 m) { return (u32)__builtin_ctzll(m); }
+VA_INLINE u32 va_cmask_pop(u64 m) { return (u32)__builtin_popcountll(m); }
+VA_INLINE u32 va_cmask_top(u64 m) { return 63u - (u32)__builtin_clzll(m); }
+#endif
+
+// Records in LDS at an odd word stride: lanes reading one field of different records then hit
+// different banks (a stride of VA_REC_STRIDE, a multiple of 32 words, would put them all on one).
+#define VA_LDS_REC_WORDS (VA_REC_STRIDE + 1u)
+
+// Software-pipelined staging of a tile's bin into LDS, G 16-byte groups of each record: while
+// chunk c is processed, the groups of chunk c + 1 (whose bin entries arrived an iteration
+// earlier) and the bin entries of chunk c + 2 are in flight in registers. Thread slot j of a
+// chunk covers group w = thread + j * NT, i.e. group w % G of the chunk's record w / G.
+template <u32 G, u32 NT>
+struct VaStager
+{
+  static constexpr u32 J = (VA_CHUNK * G + NT - 1u) / NT;
+  u32 tri[J];
+  uint4 grp[J];
+  // bin entries of chunk c (nn records)
+  VA_INLINE void load_tris(const u32 *__restrict__ bins, u32 beg, u32 nn, u32 c)
+  {
+    const u32 c0 = beg + c * VA_CHUNK;
+#pragma unroll
+    for(u32 j = 0; j < J; j++)
+    {
+      const u32 w = threadIdx.x + j * NT;
+      tri[j] = w < nn * G ? bins[c0 + w / G] : 0u;
+    }
+  }
+  // record groups of the chunk whose bin entries were loaded last
+  VA_INLINE void load_recs(const u32 *__restrict__ rec, u32 nn)
+  {
+#pragma unroll
+    for(u32 j = 0; j < J; j++)
+    {
+      const u32 w = threadIdx.x + j * NT;
+      if(w < nn * G)
+        grp[j] = ((const uint4 *)(rec + (u64)tri[j] * VA_REC_STRIDE))[w % G];
+    }
+  }
+  VA_INLINE void store(u32 *s_recw, u32 n) const
+  {
+#pragma unroll
+    for(u32 j = 0; j < J; j++)
+    {
+      const u32 w = threadIdx.x + j * NT;
+      if(w < n * G)
+      {
+        const u32 k = w / G, q = w - k * G;
+        u32 *dst = s_recw + k * VA_LDS_REC_WORDS + 4u * q;
+        dst[0] = grp[j].x;
+        dst[1] = grp[j].y;
+        dst[2] = grp[j].z;
+        dst[3] = grp[j].w;
+      }
+    }
+  }
+};
+
 // Transposes the 64x64 bit matrix held one row per lane (bit j of lane i = element (i, j)):
 // afterwards bit i of lane j is the former element (i, j). Six block-swap steps, each exchanging
 // the off-diagonal s x s blocks of every 2s x 2s block with the partner lane (lane ^ s).
@@ -523,6 +591,26 @@ VA_INLINE u64 va_transpose64(u64 x, u32 lane)
     x = (lane & s) ? ((x & ~m) | ((y >> s) & m)) : ((x & m) | ((y << s) & ~m));
   }
   return x;
+}
+
+// The same transpose when only lanes 0..31 hold rows (chunks of at most 32 records): lane i's row
+// is split into its pixel halves, lane i keeping pixels 0..31 and lane i + 32 taking pixels
+// 32..63, and each half of the wave transposes its 32x32 block in five 32-bit steps. Afterwards
+// bit i of lane j is the former element (i, j), for i < 32.
+VA_INLINE u32 va_transpose32x64(u64 x, u32 lane)
+{
+  const u32 hi = (u32)__shfl_xor((int)(u32)(x >> 32), 32);
+  u32 v = lane < 32u ? (u32)x : hi;
+  const u32 masks[5] = {0x0000FFFFu, 0x00FF00FFu, 0x0F0F0F0Fu, 0x33333333u, 0x55555555u};
+#pragma unroll
+  for(int i = 0; i < 5; i++)
+  {
+    const u32 s = 16u >> i;
+    const u32 m = masks[i];
+    const u32 y = (u32)__shfl_xor((int)v, (int)s);
+    v = (lane & s) ? ((v & ~m) | ((y >> s) & m)) : ((v & m) | ((y << s) & ~m));
+  }
+  return v;
 }
 
 VA_INLINE u32 va_wave_exclusive_sum(u32 v, u32 lane)
@@ -579,13 +667,11 @@ template <int T>
 VA_INLINE void va_raster_tile(const VaDraw *__restrict__ draws, const VaFrame *__restrict__ frames,
                               u32 nframes, const u32 *__restrict__ rec, const u32 *__restrict__ bins,
                               const u32 *__restrict__ bin_start, const u32 *__restrict__ bin_end,
-                              u32 use_fused_clear, u64 *__restrict__ part, u32 tile)
+                              u32 use_fused_clear, u64 *__restrict__ part, va_cmask *__restrict__ masks, u32 tile)
 {
   constexpr u32 SB = T / 8, NW = SB * SB, NT = NW * 64u;
   constexpr u32 Q = VA_REC_STRIDE / 4u;
-  // Records in LDS at an odd word stride: lanes reading one field of different records then hit
-  // different banks (a stride of VA_REC_STRIDE, a multiple of 32 words, would put them all on one).
-  constexpr u32 LS = VA_REC_STRIDE + 1u;
+  constexpr u32 LS = VA_LDS_REC_WORDS;
   __shared__ u32 s_recw[VA_CHUNK * LS];
   __shared__ u16 s_flags[VA_CHUNK];   // per-record fixed-function state (va_rec_flags)
   __shared__ u16 s_job[NW][64];       // jobs of one shading round: record << 6 | pixel
@@ -644,63 +730,24 @@ VA_INLINE void va_raster_tile(const VaDraw *__restrict__ draws, const VaFrame *_
   }
 
   u32 n_cov = 0, n_pass = 0, n_blend = 0, n_dw = 0;
-  // Staging is software-pipelined through registers: while chunk c is processed, the records of
-  // chunk c + 1 (whose bin entries arrived an iteration earlier) and the bin entries of chunk
-  // c + 2 are in flight. Thread slot j of a chunk covers 16-byte group w = thread + j * NT.
-  constexpr u32 JMAX = (VA_CHUNK * Q + NT - 1u) / NT;
   const u32 nchunks = (en - beg + VA_CHUNK - 1u) / VA_CHUNK;
   auto chunk_len = [&](u32 c) { return c < nchunks ? min(VA_CHUNK, en - (beg + c * VA_CHUNK)) : 0u; };
-  u32 tri_next[JMAX];
-  uint4 rec_next[JMAX];
-  auto load_tris = [&](u32 c) {
-    const u32 nn = chunk_len(c), c0 = beg + c * VA_CHUNK;
-#pragma unroll
-    for(u32 j = 0; j < JMAX; j++)
-    {
-      const u32 w = threadIdx.x + j * NT;
-      tri_next[j] = w < nn * Q ? bins[c0 + w / Q] : 0u;
-    }
-  };
-  auto load_recs = [&](u32 c) {
-    const u32 nn = chunk_len(c);
-#pragma unroll
-    for(u32 j = 0; j < JMAX; j++)
-    {
-      const u32 w = threadIdx.x + j * NT;
-      if(w < nn * Q)
-        rec_next[j] = ((const uint4 *)(rec + (u64)tri_next[j] * VA_REC_STRIDE))[w % Q];
-    }
-  };
-  load_tris(0);
-  load_recs(0);
-  load_tris(1);
+
+#if VA_FS_KILL
+  // Fragment programs that may discard decide depth writes: one pass over the bin, shading in
+  // place, in order.
+  VaStager<Q, NT> stg;
+  stg.load_tris(bins, beg, chunk_len(0), 0u);
+  stg.load_recs(rec, chunk_len(0));
+  stg.load_tris(bins, beg, chunk_len(1), 1u);
   for(u32 c = 0; c < nchunks; c++)
   {
     const u32 n = chunk_len(c);
     __syncthreads();  // the previous chunk is consumed by every wave
-#pragma unroll
-    for(u32 j = 0; j < JMAX; j++)
-    {
-      const u32 w = threadIdx.x + j * NT;
-      if(w < n * Q)
-      {
-        const u32 k = w / Q, q = w - k * Q;
-        u32 *dst = s_recw + k * LS + 4u * q;
-        dst[0] = rec_next[j].x;
-        dst[1] = rec_next[j].y;
-        dst[2] = rec_next[j].z;
-        dst[3] = rec_next[j].w;
-      }
-    }
+    stg.store(s_recw, n);
     __syncthreads();
-    for(u32 k = threadIdx.x; k < n; k += NT)
-      s_flags[k] = (u16)(s_rec[k * LS + VA_R_DRAW] >> VA_DRAW_BITS);
-    __syncthreads();
-    load_recs(c + 1);  // addresses from the bin entries loaded last iteration
-    load_tris(c + 2);
-
-#if VA_FS_KILL
-    // fragment programs that may discard decide depth writes: shade in place, in order
+    stg.load_recs(rec, chunk_len(c + 1u));  // addresses from the bin entries loaded last iteration
+    stg.load_tris(bins, beg, chunk_len(c + 2u), c + 2u);
     for(u32 k = 0; k < n; ++k)
     {
       const u32 *r = s_rec + k * LS;
@@ -729,137 +776,232 @@ VA_INLINE void va_raster_tile(const VaDraw *__restrict__ draws, const VaFrame *_
         n_dw++;
       }
     }
-#else
-    // 1. Coverage, triangle-parallel where cheap: lane k tests triangle k against this wave's
-    //    block when the bbox overlap is at most VA_SMALL pixels (same integer arithmetic as the
-    //    pixel-parallel test); larger overlaps are tested pixel-parallel, one ballot per
-    //    triangle. Either way lane k ends up holding row k of the coverage matrix
-    //    (bit p = pixel p covered by triangle k).
-    u64 row_mask = 0ull;
-    bool big = false;
-    if(lane < n)
-    {
-      const u32 *r = s_rec + lane * LS;
-      const u32 bmin = r[VA_R_BBMIN], bmax = r[VA_R_BBMAX];
-      const i32 x0 = max((i32)(bmin & 0xffffu), bx), y0 = max((i32)(bmin >> 16), by);
-      const i32 x1 = min(min((i32)(bmax & 0xffffu), bx + 8), (i32)W);
-      const i32 y1 = min(min((i32)(bmax >> 16), by + 8), (i32)H);
-      if(x0 < x1 && y0 < y1)
-      {
-        if((x1 - x0) * (y1 - y0) > VA_SMALL)
-          big = true;
-        else
-        {
-          const i32 X0 = (i32)r[VA_R_X0], Y0 = (i32)r[VA_R_Y0];
-          const u32 ABX = r[VA_R_ABX], ABY = r[VA_R_ABY], ACX = r[VA_R_ACX], ACY = r[VA_R_ACY], AR = r[VA_R_AREA];
-          for(i32 y = y0; y < y1; y++)
-            for(i32 x = x0; x < x1; x++)
-            {
-              const i32 PAx = (i32)((u32)X0 - (u32)x), PAy = (i32)((u32)Y0 - (u32)y);
-              const u32 ux = ACX * (u32)PAy - ACY * (u32)PAx;
-              const u32 uy = (u32)PAx * ABY - (u32)PAy * ABX;
-              if((i32)(AR - (ux + uy)) >= 0 && (i32)ux >= 0 && (i32)uy >= 0)
-                row_mask |= 1ull << (u32)((y - by) * 8 + (x - bx));
-            }
-        }
-      }
-    }
-    for(u64 bigs = __ballot(big); bigs; bigs &= bigs - 1ull)
-    {
-      const u32 k = (u32)__builtin_ctzll(bigs);
-      const u32 *r = s_rec + k * LS;
-      const u32 bmin = r[VA_R_BBMIN], bmax = r[VA_R_BBMAX];
-      const bool cov = inside && va_covered(r, px, py) && px >= (i32)(bmin & 0xffffu) && px < (i32)(bmax & 0xffffu) &&
-                       py >= (i32)(bmin >> 16) && py < (i32)(bmax >> 16);
-      const u64 m = __ballot(cov);
-      if(lane == k)
-        row_mask = m;
-    }
-    // 2. Transpose: lane p now holds the set of triangles (bit k) covering pixel p, so the
-    //    in-order loop below runs once per fragment of this pixel, not once per triangle.
-    u64 cov = va_transpose64(row_mask, lane);
-
-    // 3. Early depth test and depth write per pixel in primitive order (ascending k).
-    u64 pass = 0ull;
-    for(u64 todo = cov; todo; todo &= todo - 1ull)
-    {
-      const u32 k = (u32)__builtin_ctzll(todo);
-      const u32 f = s_flags[k];
-      const u32 op = f & 7u;
-      n_cov++;
-      bool passed = true;
-      float z = 0.0f;
-      if(has_depth && (op != 7u || (f & VA_F_DWRITE)))
-      {
-        va_cov_depth(s_rec + k * LS, px, py, &z);
-        if(op != 7u)
-          passed = va_depth_pass(op, z, dep);
-      }
-      if(passed)
-      {
-        pass |= 1ull << k;
-        n_pass++;
-        n_blend += (f >> 4) & 1u;
-        if((f & VA_F_DWRITE) && has_depth)
-        {
-          dep = z;
-          n_dw++;
-        }
-      }
-    }
-
-    // 4. Shade every passing fragment in rounds of 64, jobs in pixel-major order (pixel p owns
-    //    jobs [off_p, off_p + count_p), in ascending k): each pixel writes its jobs of the round
-    //    to the job list, the wave shades the list 64 wide, then each pixel blends its own
-    //    results of the round in order.
-    const u32 cnt = (u32)__popcll(pass);
-    const u32 off = va_wave_exclusive_sum(cnt, lane);
-    const u32 total = (u32)__builtin_amdgcn_readlane((int)(off + cnt), 63);
-    u64 wrem = pass, brem = pass;  // write and blend cursors over this pixel's jobs
-    u32 wnext = off, bnext = off;
-    const u32 mine_end = off + cnt;
-    for(u32 base = 0; base < total; base += 64u)
-    {
-      const u32 stop = min(mine_end, base + 64u);
-      // each pixel lists its jobs of this round, in ascending record order
-      for(; wnext < stop; ++wnext)
-      {
-        const u32 k = (u32)__builtin_ctzll(wrem);
-        wrem &= wrem - 1ull;
-        s_job[wave][wnext - base] = (u16)((k << 6) | lane);
-      }
-      va_wave_sync();
-      if(base + lane < total)
-      {
-        const u32 job = s_job[wave][lane];
-        const u32 k = job >> 6, p = job & 63u;
-        const u32 *r = s_rec + k * LS;
-        const i32 x = bx + (i32)(p & 7u), y = by + (i32)(p >> 3);
-        float b[3][4];
-        va_bary(r, x, y, b[0], nullptr);
-#if VA_FS_DERIV
-        va_bary(r, x + 1, y, b[1], nullptr);
-        va_bary(r, x, y + 1, b[2], nullptr);
-#endif
-        const VaDraw &d = draws[r[VA_R_DRAW] & VA_DRAW_MASK];
-        float o[4];
-        va_fs(d.slot, d, (const float *)(r + VA_REC_HDR), b, o);
-        s_res[wave][lane] = make_float4(o[0], o[1], o[2], o[3]);
-      }
-      va_wave_sync();
-      for(; bnext < stop; ++bnext)
-      {
-        const u32 k = (u32)__builtin_ctzll(brem);
-        brem &= brem - 1ull;
-        const float4 v = s_res[wave][bnext - base];
-        float o[4] = {v.x, v.y, v.z, v.w};
-        va_blend_store_flags(s_flags[k], hdr, o, c0, c1);
-      }
-      // the next round rewrites s_job and s_res
-      va_wave_sync();
-    }
-#endif
   }
+#else
+  // Two passes over the bin. Pass A (visibility) runs coverage, the early depth test and depth
+  // writes for every fragment in primitive order, keeps each chunk's per-pixel pass mask in
+  // `masks`, and finds per pixel the last passing fragment of a draw without blending. That
+  // fragment's store replaces the pixel's B, G and R bytes (every half of an RGBA16F pixel)
+  // whatever was there, and fragment programs here cannot discard, so the passing fragments
+  // before it cannot change the final colour. Pass B (shading) runs the fragment program only
+  // for that fragment and the ones after it, and blends them in order. The depth buffer and
+  // every counter are final after pass A.
+  va_cmask *mk = masks + (u64)(NW * 64u) * (beg / VA_CHUNK + tile) + wave * 64u + lane;
+  u32 cut_c = 0xffffffffu, cut_k = 0u, first_c = 0xffffffffu;
+  {
+    // the header words of each record suffice for coverage and depth
+    constexpr u32 HQ = (VA_REC_HDR + 3u) / 4u < Q ? (VA_REC_HDR + 3u) / 4u : Q;
+    VaStager<HQ, NT> stg;
+    stg.load_tris(bins, beg, chunk_len(0), 0u);
+    stg.load_recs(rec, chunk_len(0));
+    stg.load_tris(bins, beg, chunk_len(1), 1u);
+    for(u32 c = 0; c < nchunks; c++)
+    {
+      const u32 n = chunk_len(c);
+      __syncthreads();  // the previous chunk is consumed by every wave
+      stg.store(s_recw, n);
+      __syncthreads();
+      stg.load_recs(rec, chunk_len(c + 1u));
+      stg.load_tris(bins, beg, chunk_len(c + 2u), c + 2u);
+
+      // 1. Coverage, triangle-parallel where cheap: lane k tests triangle k against this wave's
+      //    block when the bbox overlap is at most VA_SMALL pixels (same integer arithmetic as the
+      //    pixel-parallel test); larger overlaps are tested pixel-parallel, one ballot per
+      //    triangle. Either way lane k ends up holding row k of the coverage matrix
+      //    (bit p = pixel p covered by triangle k).
+      u64 row_mask = 0ull;
+      bool big = false;
+      u32 fl = 0u;
+      if(lane < n)
+      {
+        const u32 *r = s_rec + lane * LS;
+        fl = r[VA_R_DRAW] >> VA_DRAW_BITS;
+        const u32 bmin = r[VA_R_BBMIN], bmax = r[VA_R_BBMAX];
+        const i32 x0 = max((i32)(bmin & 0xffffu), bx), y0 = max((i32)(bmin >> 16), by);
+        const i32 x1 = min(min((i32)(bmax & 0xffffu), bx + 8), (i32)W);
+        const i32 y1 = min(min((i32)(bmax >> 16), by + 8), (i32)H);
+        if(x0 < x1 && y0 < y1)
+        {
+          if((x1 - x0) * (y1 - y0) > VA_SMALL)
+            big = true;
+          else
+          {
+            const i32 X0 = (i32)r[VA_R_X0], Y0 = (i32)r[VA_R_Y0];
+            const u32 ABX = r[VA_R_ABX], ABY = r[VA_R_ABY], ACX = r[VA_R_ACX], ACY = r[VA_R_ACY], AR = r[VA_R_AREA];
+            for(i32 y = y0; y < y1; y++)
+              for(i32 x = x0; x < x1; x++)
+              {
+                const i32 PAx = (i32)((u32)X0 - (u32)x), PAy = (i32)((u32)Y0 - (u32)y);
+                const u32 ux = ACX * (u32)PAy - ACY * (u32)PAx;
+                const u32 uy = (u32)PAx * ABY - (u32)PAy * ABX;
+                if((i32)(AR - (ux + uy)) >= 0 && (i32)ux >= 0 && (i32)uy >= 0)
+                  row_mask |= 1ull << (u32)((y - by) * 8 + (x - bx));
+              }
+          }
+        }
+      }
+      // records of draws that blend (bit k = record k)
+      const va_cmask blends = (va_cmask)__ballot((fl & VA_F_BLEND) != 0u);
+      for(u64 bigs = __ballot(big); bigs; bigs &= bigs - 1ull)
+      {
+        const u32 k = (u32)__builtin_ctzll(bigs);
+        const u32 *r = s_rec + k * LS;
+        const u32 bmin = r[VA_R_BBMIN], bmax = r[VA_R_BBMAX];
+        const bool cov = inside && va_covered(r, px, py) && px >= (i32)(bmin & 0xffffu) && px < (i32)(bmax & 0xffffu) &&
+                         py >= (i32)(bmin >> 16) && py < (i32)(bmax >> 16);
+        const u64 m = __ballot(cov);
+        if(lane == k)
+          row_mask = m;
+      }
+      // 2. Transpose: lane p now holds the set of records (bit k) covering pixel p, so the
+      //    in-order loop below runs once per fragment of this pixel, not once per triangle.
+#if VA_CHUNK <= 32
+      const va_cmask cov = va_transpose32x64(row_mask, lane);
+#else
+      const va_cmask cov = va_transpose64(row_mask, lane);
+#endif
+
+      // 3. Early depth test and depth write per pixel in primitive order (ascending k).
+      va_cmask pass = 0;
+      for(va_cmask todo = cov; todo; todo &= todo - 1u)
+      {
+        const u32 k = va_cmask_ctz(todo);
+        const u32 f = s_rec[k * LS + VA_R_DRAW] >> VA_DRAW_BITS;
+        const u32 op = f & 7u;
+        n_cov++;
+        bool passed = true;
+        float z = 0.0f;
+        if(has_depth && (op != 7u || (f & VA_F_DWRITE)))
+        {
+          va_cov_depth(s_rec + k * LS, px, py, &z);
+          if(op != 7u)
+            passed = va_depth_pass(op, z, dep);
+        }
+        if(passed)
+        {
+          pass |= (va_cmask)1u << k;
+          n_pass++;
+          n_blend += (f >> 4) & 1u;
+          if((f & VA_F_DWRITE) && has_depth)
+          {
+            dep = z;
+            n_dw++;
+          }
+        }
+      }
+      mk[(u64)c * (NW * 64u)] = pass;
+      const va_cmask opaque = pass & ~blends;
+      if(opaque)
+      {
+        cut_c = c;
+        cut_k = va_cmask_top(opaque);
+      }
+      if(pass && first_c == 0xffffffffu)
+        first_c = c;
+    }
+  }
+
+  // Pass B. A pixel's shading starts at its cut (chunk cut_c, record cut_k) or, with no
+  // fragment of a non-blending draw, at its first passing fragment; the tile's walk starts at
+  // the earliest of these.
+  u32 start_c = cut_c != 0xffffffffu ? cut_c : first_c;
+  start_c = min(start_c, (u32)__shfl_xor((int)start_c, 1));
+  start_c = min(start_c, (u32)__shfl_xor((int)start_c, 2));
+  start_c = min(start_c, (u32)__shfl_xor((int)start_c, 4));
+  start_c = min(start_c, (u32)__shfl_xor((int)start_c, 8));
+  start_c = min(start_c, (u32)__shfl_xor((int)start_c, 16));
+  start_c = min(start_c, (u32)__shfl_xor((int)start_c, 32));
+  if(NW > 1u)
+  {
+    __shared__ u32 s_start;
+    if(threadIdx.x == 0u)
+      s_start = 0xffffffffu;
+    __syncthreads();
+    if(lane == 0u)
+      atomicMin(&s_start, start_c);
+    __syncthreads();
+    start_c = s_start;
+  }
+  start_c = __builtin_amdgcn_readfirstlane(start_c);
+  if(start_c < nchunks)
+  {
+    VaStager<Q, NT> stg;
+    stg.load_tris(bins, beg, chunk_len(start_c), start_c);
+    stg.load_recs(rec, chunk_len(start_c));
+    stg.load_tris(bins, beg, chunk_len(start_c + 1u), start_c + 1u);
+    va_cmask m_next = mk[(u64)start_c * (NW * 64u)];
+    for(u32 c = start_c; c < nchunks; c++)
+    {
+      const u32 n = chunk_len(c);
+      const va_cmask m = m_next;
+      __syncthreads();  // the previous chunk is consumed by every wave
+      stg.store(s_recw, n);
+      __syncthreads();
+      for(u32 k = threadIdx.x; k < n; k += NT)
+        s_flags[k] = (u16)(s_rec[k * LS + VA_R_DRAW] >> VA_DRAW_BITS);
+      __syncthreads();
+      stg.load_recs(rec, chunk_len(c + 1u));
+      stg.load_tris(bins, beg, chunk_len(c + 2u), c + 2u);
+      if(c + 1u < nchunks)
+        m_next = mk[(u64)(c + 1u) * (NW * 64u)];
+      // this pixel's fragments to shade in this chunk
+      const va_cmask alive = cut_c == 0xffffffffu || c > cut_c ? m
+                             : c == cut_c                     ? m & (va_cmask)(~(va_cmask)0 << cut_k)
+                                                              : (va_cmask)0;
+
+      // 4. Shade every alive fragment in rounds of 64, jobs in pixel-major order (pixel p owns
+      //    jobs [off_p, off_p + count_p), in ascending k): each pixel writes its jobs of the round
+      //    to the job list, the wave shades the list 64 wide, then each pixel blends its own
+      //    results of the round in order.
+      const u32 cnt = va_cmask_pop(alive);
+      const u32 off = va_wave_exclusive_sum(cnt, lane);
+      const u32 total = (u32)__builtin_amdgcn_readlane((int)(off + cnt), 63);
+      va_cmask wrem = alive, brem = alive;  // write and blend cursors over this pixel's jobs
+      u32 wnext = off, bnext = off;
+      const u32 mine_end = off + cnt;
+      for(u32 base = 0; base < total; base += 64u)
+      {
+        const u32 stop = min(mine_end, base + 64u);
+        // each pixel lists its jobs of this round, in ascending record order
+        for(; wnext < stop; ++wnext)
+        {
+          const u32 k = va_cmask_ctz(wrem);
+          wrem &= wrem - 1u;
+          s_job[wave][wnext - base] = (u16)((k << 6) | lane);
+        }
+        va_wave_sync();
+        if(base + lane < total)
+        {
+          const u32 job = s_job[wave][lane];
+          const u32 k = job >> 6, p = job & 63u;
+          const u32 *r = s_rec + k * LS;
+          const i32 x = bx + (i32)(p & 7u), y = by + (i32)(p >> 3);
+          float b[3][4];
+          va_bary(r, x, y, b[0], nullptr);
+#if VA_FS_DERIV
+          va_bary(r, x + 1, y, b[1], nullptr);
+          va_bary(r, x, y + 1, b[2], nullptr);
+#endif
+          const VaDraw &d = draws[r[VA_R_DRAW] & VA_DRAW_MASK];
+          float o[4];
+          va_fs(d.slot, d, (const float *)(r + VA_REC_HDR), b, o);
+          s_res[wave][lane] = make_float4(o[0], o[1], o[2], o[3]);
+        }
+        va_wave_sync();
+        for(; bnext < stop; ++bnext)
+        {
+          const u32 k = va_cmask_ctz(brem);
+          brem &= brem - 1u;
+          const float4 v = s_res[wave][bnext - base];
+          float o[4] = {v.x, v.y, v.z, v.w};
+          va_blend_store_flags(s_flags[k], hdr, o, c0, c1);
+        }
+        // the next round rewrites s_job and s_res
+        va_wave_sync();
+      }
+    }
+  }
+#endif
 
   if(inside)
   {
@@ -901,13 +1043,13 @@ VA_INLINE u32 va_xcd_tile(u32 b, u32 n)
       const VaDraw *__restrict__ draws, const VaFrame *__restrict__ frames, u32 nframes,                  \
       const u32 *__restrict__ rec, const u32 *__restrict__ bins, const u32 *__restrict__ bin_start,        \
       const u32 *__restrict__ bin_end, u32 ntiles, u32 use_fused_clear, u64 *__restrict__ part,            \
-      const u32 *__restrict__ pair_total, u32 pair_cap, u64 *__restrict__ stamp)                            \
+      const u32 *__restrict__ pair_total, u32 pair_cap, va_cmask *__restrict__ masks, u64 *__restrict__ stamp) \
   {                                                                                                         \
     const u32 tile = va_xcd_tile(blockIdx.x, ntiles);                                                       \
     if(blockIdx.x == 0u && threadIdx.x == 0u)                                                               \
       *stamp = (u64)wall_clock64();                                                                         \
     if(tile < ntiles && *pair_total <= pair_cap)                                                            \
-      va_raster_tile<T>(draws, frames, nframes, rec, bins, bin_start, bin_end, use_fused_clear, part, tile); \
+      va_raster_tile<T>(draws, frames, nframes, rec, bins, bin_start, bin_end, use_fused_clear, part, masks, tile); \
   }
 
 VA_RASTER_KERNEL(8)
@@ -921,7 +1063,8 @@ VA_RASTER_KERNEL(32)
       const VaDraw *__restrict__ draws, const VaFrame *__restrict__ frames, u32 nframes,                    \
       const u32 *__restrict__ rec, const u32 *__restrict__ bins, const u32 *__restrict__ bin_start,          \
       const u32 *__restrict__ bin_end, u32 ntiles, u32 use_fused_clear, u64 *__restrict__ part,              \
-      const u32 *__restrict__ pair_total, u32 pair_cap, u32 *__restrict__ tile_counter, u64 *__restrict__ stamp) \
+      const u32 *__restrict__ pair_total, u32 pair_cap, va_cmask *__restrict__ masks, u32 *__restrict__ tile_counter, \
+      u64 *__restrict__ stamp)                                                                                \
   {                                                                                                           \
     __shared__ u32 s_next;                                                                                    \
     if(blockIdx.x == 0u && threadIdx.x == 0u)                                                                 \
@@ -937,7 +1080,7 @@ VA_RASTER_KERNEL(32)
       __syncthreads();                                                                                        \
       if(tile >= ntiles)                                                                                      \
         break;                                                                                                \
-      va_raster_tile<T>(draws, frames, nframes, rec, bins, bin_start, bin_end, use_fused_clear, part, tile);   \
+      va_raster_tile<T>(draws, frames, nframes, rec, bins, bin_start, bin_end, use_fused_clear, part, masks, tile); \
     }                                                                                                         \
   }
 

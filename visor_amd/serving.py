@@ -23,7 +23,7 @@ from .vk import Handle
 
 TARGET_SIZES = ((256, 256), (512, 512), (1024, 1024), (1920, 1080))
 TEXTURE_SIZES = (256, 512, 1024, 2048)
-MESH_GRIDS = ((10, 25), (20, 50), (40, 100), (80, 200), (160, 400), (250, 500))  # 2*s*t triangles
+MESH_GRIDS = ((10, 25), (20, 50), (40, 100), (80, 200), (160, 400), (250, 1000))  # 2*s*t triangles
 
 
 class Client:
@@ -38,8 +38,9 @@ class Client:
 
 class ServingWorld:
     def __init__(self, ctx, nclients=64, seed=4, target_sizes=TARGET_SIZES, texture_sizes=TEXTURE_SIZES,
-                 mesh_grids=MESH_GRIDS, tris_range=(1000, 500000), draws_range=(1, 32)):
+                 mesh_grids=MESH_GRIDS, tris_range=(1000, 500000), draws_range=(1, 32), host_visible_depth=False):
         self.ctx = ctx
+        self.host_visible_depth = host_visible_depth
         rng = random.Random(seed)
         nrng = scenes.Rng(seed)
         # mesh pool: one vertex and one index buffer, each mesh a contiguous index range
@@ -70,9 +71,12 @@ class ServingWorld:
             w, h = target_sizes[rng.randrange(len(target_sizes))]
             nd = rng.randint(draws_range[0], draws_range[1])
             total = math.exp(rng.uniform(lo, hi))
-            per = total / nd
-            k = min(range(len(self.meshes)), key=lambda m: abs(math.log(self.meshes[m][1] / 3) - math.log(per)))
-            first, count = self.meshes[k]
+            # each draw takes `per` triangles (a leading index range) of the smallest mesh that has
+            # that many, so the request's total stays within tris_range
+            per = min(max(round(total / nd), -(-tris_range[0] // nd)), tris_range[1] // nd)
+            first, mesh_count = min((m for m in self.meshes if m[1] >= 3 * per), key=lambda m: m[1],
+                                    default=self.meshes[-1])
+            count = min(3 * per, mesh_count)
             proj = scenes.perspective(math.radians(60), w / h, 0.5, 20.0)
             draws = []
             for _ in range(nd):
@@ -106,7 +110,8 @@ class ServingWorld:
         b = self.base
         for c in self.clients:
             c.color, c.cmem, c.cview, _ = ctx.image(c.width, c.height, vk.FMT["B8G8R8A8_UNORM"], host_visible=True)
-            c.depth, c.dmem, c.dview, _ = ctx.image(c.width, c.height, vk.FMT["D32_SFLOAT"], host_visible=False)
+            c.depth, c.dmem, c.dview, _ = ctx.image(c.width, c.height, vk.FMT["D32_SFLOAT"],
+                                                    host_visible=self.host_visible_depth)
             views = (Handle * 2)(c.cview, c.dview)
             fci = vk.FramebufferCreateInfo(renderPass=b.rp, attachmentCount=2, pAttachments=views, width=c.width,
                                            height=c.height, layers=1)
@@ -139,6 +144,13 @@ class ServingWorld:
         n = c.width * c.height * 4
         p = self.ctx.map(c.cmem, n)
         return np.frombuffer((C.c_uint8 * n).from_address(p), dtype=np.uint8).copy().reshape(c.height, c.width, 4)
+
+
+    def read_depth(self, c):
+        """The client's depth target (a world built with host_visible_depth=True)."""
+        n = c.width * c.height * 4
+        p = self.ctx.map(c.dmem, n)
+        return np.frombuffer((C.c_uint8 * n).from_address(p), dtype=np.float32).copy().reshape(c.height, c.width)
 
 
 class Server:
@@ -215,3 +227,24 @@ def run_clients(world, server, rate_rps, seconds, seed=0):
     if errors:
         raise errors[0]
     return server.stats(), wall
+
+
+def run_sequential(world, ctx, seconds):
+    """The same clients' requests through the ordinary queue, one at a time: a synchronous
+    vkQueueSubmit per request (the reference's blocking submit, cmd_exec.cpp:187-201), round-robin
+    over the clients for `seconds`. Returns completed requests, covered fragments, wall time and
+    the per-request latencies (submit -> the target's bytes in host memory) in ms."""
+    lat, frags, done = [], 0, 0
+    t0 = time.perf_counter()
+    t_end = t0 + seconds
+    while time.perf_counter() < t_end:
+        for c in world.clients:
+            t = time.perf_counter()
+            ctx.icd.check(ctx.submit([c.cb]), "vkQueueSubmit")
+            ctx.icd.check(ctx.icd.vkQueueWaitIdle(ctx.queue), "vkQueueWaitIdle")
+            lat.append((time.perf_counter() - t) * 1e3)
+            frags += ctx.stats(0)["pixels_written"]
+            done += 1
+            if time.perf_counter() >= t_end:
+                break
+    return done, frags, time.perf_counter() - t0, lat
