@@ -17,6 +17,13 @@ hipError_t va_scan_u32(uint32_t *temp, const uint32_t *in, uint32_t *out, uint32
 size_t va_sort_temp_words(uint32_t cap);
 hipError_t va_sort_pairs(uint32_t *temp, uint32_t *keys, uint32_t *vals, uint32_t *keys_alt, uint32_t *vals_alt,
                          const uint32_t *n_dev, uint32_t cap, int bits, int *result_in_alt, hipStream_t stream);
+/* va_sort_pairs without a workspace: the histogram and scan words come from a device array of
+ * the library sized for VA_SORT_NOALLOC_MAX_PAIRS pairs (4 MiB), so the call allocates nothing.
+ * cap above the maximum returns hipErrorInvalidValue. Calls on one device share the array:
+ * calls that overlap on one device race. */
+#define VA_SORT_NOALLOC_MAX_PAIRS (1u << 24)
+hipError_t va_sort_pairs_noalloc(uint32_t *keys, uint32_t *vals, uint32_t *keys_alt, uint32_t *vals_alt,
+                                 const uint32_t *n_dev, uint32_t cap, int bits, int *result_in_alt, hipStream_t stream);
 hipError_t va_fill_u32_async(void *p, uint64_t count, uint32_t value, hipStream_t stream);
 hipError_t va_fill_u64_async(void *p, uint64_t count, uint32_t lo, uint32_t hi, hipStream_t stream);
 

@@ -5,6 +5,8 @@
 // into a HIP graph.
 #include <hip/hip_runtime.h>
 
+#include <mutex>
+
 #include "runtime_kernels.h"
 #include "texel_decode.h"
 
@@ -333,6 +335,61 @@ extern "C" hipError_t va_sort_pairs(uint32_t *temp, uint32_t *keys, uint32_t *va
     *result_in_alt ^= 1;
   }
   return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------------------------
+// The same sort without a workspace argument and without allocating: the histogram and scan
+// words live in `scratch`, a file-scope device array sized for VA_SORT_NOALLOC_MAX_PAIRS pairs.
+// Host code reaches the array through the address hipGetSymbolAddress returns for the current
+// device (the array's name is only a host-side stand-in), looked up once per device on the first
+// call there. Every call on one device shares that device's array; nothing in it needs clearing
+// (the histogram kernel writes every word the scan and scatter read), and calls that overlap on
+// one device (two streams) race on it.
+
+static constexpr size_t kNoallocSortWords =
+    256u * ((VA_SORT_NOALLOC_MAX_PAIRS + RK_SORT_B - 1) / RK_SORT_B) +
+    (256u * ((VA_SORT_NOALLOC_MAX_PAIRS + RK_SORT_B - 1) / RK_SORT_B) + RK_SCAN_B - 1) / RK_SCAN_B + 64u;
+static __device__ __align__(256) uint32_t scratch[kNoallocSortWords];
+
+#define RK_MAX_DEVICES 64
+static std::mutex g_scratch_mu;
+static uint32_t *g_scratch_ptr[RK_MAX_DEVICES];
+
+static hipError_t rk_scratch_for_current_device(uint32_t **out)
+{
+  int dev = -1;
+  hipError_t e = hipGetDevice(&dev);
+  if(e != hipSuccess)
+    return e;
+  if(dev < 0 || dev >= RK_MAX_DEVICES)
+    return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if(!g_scratch_ptr[dev])
+  {
+    void *p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if(e != hipSuccess)
+      return e;
+    g_scratch_ptr[dev] = (uint32_t *)p;
+  }
+  *out = g_scratch_ptr[dev];
+  return hipSuccess;
+}
+
+extern "C" hipError_t va_sort_pairs_noalloc(uint32_t *keys, uint32_t *vals, uint32_t *keys_alt, uint32_t *vals_alt,
+                                            const uint32_t *n_dev, uint32_t cap, int bits, int *result_in_alt,
+                                            hipStream_t stream)
+{
+  *result_in_alt = 0;
+  if(cap > VA_SORT_NOALLOC_MAX_PAIRS)
+    return hipErrorInvalidValue;
+  if(va_sort_temp_words(cap) > kNoallocSortWords)  // holds for every cap within the maximum
+    return hipErrorInvalidValue;
+  uint32_t *temp = nullptr;
+  const hipError_t e = rk_scratch_for_current_device(&temp);
+  if(e != hipSuccess)
+    return e;
+  return va_sort_pairs(temp, keys, vals, keys_alt, vals_alt, n_dev, cap, bits, result_in_alt, stream);
 }
 
 __global__ __launch_bounds__(256) void va_fill32_kernel(uint32_t *p, uint64_t n, uint32_t v)
