@@ -395,14 +395,26 @@ VA_INLINE u32 va_unorm_out(float v)
 VA_INLINE u32 va_half_bits(float f) { _Float16 h = (_Float16)f; return (u32)__builtin_bit_cast(u16, h); }
 VA_INLINE float va_half_val(u32 b) { return (float)__builtin_bit_cast(_Float16, (u16)b); }
 
+// Edge-function products, modulo 2^32 as the reference's int32 arithmetic wraps. With S (every
+// record of the chunk has window coordinates within +-8191, see va_rec_small, and the target is at
+// most 8192 pixels per side) each factor fits in 24 signed bits and each product is below 2^28, so
+// the 24-bit multiply (full rate, where the 32-bit one issues at a quarter of the rate) gives the
+// same value.
+template <bool S>
+VA_INLINE u32 va_emul(u32 a, u32 b)
+{
+  return S ? (u32)__mul24((int)a, (int)b) : a * b;
+}
+
 // Perspective-correct barycentrics at integer corner (x, y) from the record
 // (rasterizer.cpp:282-310, 551-588). Returns coverage; writes the pre-perspective depth.
+template <bool S = false>
 VA_INLINE bool va_bary(const u32 *__restrict__ r, i32 x, i32 y, float b[4], float *depth)
 {
   const i32 PAx = (i32)(r[VA_R_X0] - (u32)x);
   const i32 PAy = (i32)(r[VA_R_Y0] - (u32)y);
-  const u32 ux = r[VA_R_ACX] * (u32)PAy - r[VA_R_ACY] * (u32)PAx;
-  const u32 uy = (u32)PAx * r[VA_R_ABY] - (u32)PAy * r[VA_R_ABX];
+  const u32 ux = va_emul<S>(r[VA_R_ACX], (u32)PAy) - va_emul<S>(r[VA_R_ACY], (u32)PAx);
+  const u32 uy = va_emul<S>((u32)PAx, r[VA_R_ABY]) - va_emul<S>((u32)PAy, r[VA_R_ABX]);
   const i32 b0 = (i32)(r[VA_R_AREA] - (ux + uy)), b1 = (i32)ux, b2 = (i32)uy;
   const bool covered = b0 >= 0 && b1 >= 0 && b2 >= 0;
   const float invarea = va_u2f(r[VA_R_INVAREA]);
@@ -573,6 +585,22 @@ struct VaStager
   }
 };
 
+// True when the 24-bit multiply of va_emul is exact for this record at pixel coordinates below
+// 2^22: |X0|, |Y0| < 2^22 and edge coefficients within 24 signed bits keep every factor of the
+// edge products (the coefficients and X0 - x, Y0 - y) within 24 signed bits.
+VA_INLINE bool va_rec_small(const u32 *__restrict__ r)
+{
+  const u32 h22 = 1u << 22, h23 = 1u << 23;
+  return r[VA_R_X0] + h22 < 2u * h22 && r[VA_R_Y0] + h22 < 2u * h22 && r[VA_R_ABX] + h23 < 2u * h23 &&
+         r[VA_R_ABY] + h23 < 2u * h23 && r[VA_R_ACX] + h23 < 2u * h23 && r[VA_R_ACY] + h23 < 2u * h23;
+}
+
+template <bool B>
+struct va_bool
+{
+  static constexpr bool value = B;
+};
+
 // Transposes the 64x64 bit matrix held one row per lane (bit j of lane i = element (i, j)):
 // afterwards bit i of lane j is the former element (i, j). Six block-swap steps, each exchanging
 // the off-diagonal s x s blocks of every 2s x 2s block with the partner lane (lane ^ s).
@@ -627,12 +655,13 @@ VA_INLINE u32 va_wave_exclusive_sum(u32 v, u32 lane)
 }
 
 // Coverage only (the integer part of va_bary).
+template <bool S = false>
 VA_INLINE bool va_covered(const u32 *__restrict__ r, i32 x, i32 y)
 {
   const i32 PAx = (i32)(r[VA_R_X0] - (u32)x);
   const i32 PAy = (i32)(r[VA_R_Y0] - (u32)y);
-  const u32 ux = r[VA_R_ACX] * (u32)PAy - r[VA_R_ACY] * (u32)PAx;
-  const u32 uy = (u32)PAx * r[VA_R_ABY] - (u32)PAy * r[VA_R_ABX];
+  const u32 ux = va_emul<S>(r[VA_R_ACX], (u32)PAy) - va_emul<S>(r[VA_R_ACY], (u32)PAx);
+  const u32 uy = va_emul<S>((u32)PAx, r[VA_R_ABY]) - va_emul<S>((u32)PAy, r[VA_R_ABX]);
   return (i32)(r[VA_R_AREA] - (ux + uy)) >= 0 && (i32)ux >= 0 && (i32)uy >= 0;
 }
 
@@ -647,12 +676,13 @@ VA_INLINE bool va_covered(const u32 *__restrict__ r, i32 x, i32 y)
 // of every fragment are those of the reference; only the lane that runs a fragment program
 // changes.
 // Coverage and interpolated depth only (the first half of va_bary, same operations).
+template <bool S = false>
 VA_INLINE bool va_cov_depth(const u32 *__restrict__ r, i32 x, i32 y, float *depth)
 {
   const i32 PAx = (i32)(r[VA_R_X0] - (u32)x);
   const i32 PAy = (i32)(r[VA_R_Y0] - (u32)y);
-  const u32 ux = r[VA_R_ACX] * (u32)PAy - r[VA_R_ACY] * (u32)PAx;
-  const u32 uy = (u32)PAx * r[VA_R_ABY] - (u32)PAy * r[VA_R_ABX];
+  const u32 ux = va_emul<S>(r[VA_R_ACX], (u32)PAy) - va_emul<S>(r[VA_R_ACY], (u32)PAx);
+  const u32 uy = va_emul<S>((u32)PAx, r[VA_R_ABY]) - va_emul<S>((u32)PAy, r[VA_R_ABX]);
   const i32 b0 = (i32)(r[VA_R_AREA] - (ux + uy)), b1 = (i32)ux, b2 = (i32)uy;
   const float invarea = va_u2f(r[VA_R_INVAREA]);
   float n0 = (float)b0, n1 = (float)b1, n2 = (float)b2;
@@ -810,12 +840,13 @@ VA_INLINE void va_raster_tile(const VaDraw *__restrict__ draws, const VaFrame *_
       //    triangle. Either way lane k ends up holding row k of the coverage matrix
       //    (bit p = pixel p covered by triangle k).
       u64 row_mask = 0ull;
-      bool big = false;
+      bool big = false, wide = false;
       u32 fl = 0u;
       if(lane < n)
       {
         const u32 *r = s_rec + lane * LS;
         fl = r[VA_R_DRAW] >> VA_DRAW_BITS;
+        wide = !va_rec_small(r);
         const u32 bmin = r[VA_R_BBMIN], bmax = r[VA_R_BBMAX];
         const i32 x0 = max((i32)(bmin & 0xffffu), bx), y0 = max((i32)(bmin >> 16), by);
         const i32 x1 = min(min((i32)(bmax & 0xffffu), bx + 8), (i32)W);
@@ -826,33 +857,49 @@ VA_INLINE void va_raster_tile(const VaDraw *__restrict__ draws, const VaFrame *_
             big = true;
           else
           {
-            const i32 X0 = (i32)r[VA_R_X0], Y0 = (i32)r[VA_R_Y0];
+            // the edge functions at (x0, y0), then stepped: x + 1 adds ACY to ux and -ABY to uy,
+            // y + 1 adds -ACX and ABX (the same values modulo 2^32 as evaluating each pixel)
             const u32 ABX = r[VA_R_ABX], ABY = r[VA_R_ABY], ACX = r[VA_R_ACX], ACY = r[VA_R_ACY], AR = r[VA_R_AREA];
+            const u32 PAx = r[VA_R_X0] - (u32)x0, PAy = r[VA_R_Y0] - (u32)y0;
+            u32 ux_row = ACX * PAy - ACY * PAx;
+            u32 uy_row = PAx * ABY - PAy * ABX;
             for(i32 y = y0; y < y1; y++)
+            {
+              u32 ux = ux_row, uy = uy_row;
               for(i32 x = x0; x < x1; x++)
               {
-                const i32 PAx = (i32)((u32)X0 - (u32)x), PAy = (i32)((u32)Y0 - (u32)y);
-                const u32 ux = ACX * (u32)PAy - ACY * (u32)PAx;
-                const u32 uy = (u32)PAx * ABY - (u32)PAy * ABX;
                 if((i32)(AR - (ux + uy)) >= 0 && (i32)ux >= 0 && (i32)uy >= 0)
                   row_mask |= 1ull << (u32)((y - by) * 8 + (x - bx));
+                ux += ACY;
+                uy -= ABY;
               }
+              ux_row -= ACX;
+              uy_row += ABX;
+            }
           }
         }
       }
       // records of draws that blend (bit k = record k)
       const va_cmask blends = (va_cmask)__ballot((fl & VA_F_BLEND) != 0u);
-      for(u64 bigs = __ballot(big); bigs; bigs &= bigs - 1ull)
-      {
-        const u32 k = (u32)__builtin_ctzll(bigs);
-        const u32 *r = s_rec + k * LS;
-        const u32 bmin = r[VA_R_BBMIN], bmax = r[VA_R_BBMAX];
-        const bool cov = inside && va_covered(r, px, py) && px >= (i32)(bmin & 0xffffu) && px < (i32)(bmax & 0xffffu) &&
-                         py >= (i32)(bmin >> 16) && py < (i32)(bmax >> 16);
-        const u64 m = __ballot(cov);
-        if(lane == k)
-          row_mask = m;
-      }
+      // every record of the chunk allows the 24-bit edge products (a uniform branch below)
+      const bool chunk_small = W < (1u << 21) && H < (1u << 21) && __ballot(wide) == 0ull;
+      auto big_tests = [&](auto sm) {
+        for(u64 bigs = __ballot(big); bigs; bigs &= bigs - 1ull)
+        {
+          const u32 k = (u32)__builtin_ctzll(bigs);
+          const u32 *r = s_rec + k * LS;
+          const u32 bmin = r[VA_R_BBMIN], bmax = r[VA_R_BBMAX];
+          const bool cov = inside && va_covered<decltype(sm)::value>(r, px, py) && px >= (i32)(bmin & 0xffffu) &&
+                           px < (i32)(bmax & 0xffffu) && py >= (i32)(bmin >> 16) && py < (i32)(bmax >> 16);
+          const u64 m = __ballot(cov);
+          if(lane == k)
+            row_mask = m;
+        }
+      };
+      if(chunk_small)
+        big_tests(va_bool<true>());
+      else
+        big_tests(va_bool<false>());
       // 2. Transpose: lane p now holds the set of records (bit k) covering pixel p, so the
       //    in-order loop below runs once per fragment of this pixel, not once per triangle.
 #if VA_CHUNK <= 32
@@ -863,32 +910,38 @@ VA_INLINE void va_raster_tile(const VaDraw *__restrict__ draws, const VaFrame *_
 
       // 3. Early depth test and depth write per pixel in primitive order (ascending k).
       va_cmask pass = 0;
-      for(va_cmask todo = cov; todo; todo &= todo - 1u)
-      {
-        const u32 k = va_cmask_ctz(todo);
-        const u32 f = s_rec[k * LS + VA_R_DRAW] >> VA_DRAW_BITS;
-        const u32 op = f & 7u;
-        n_cov++;
-        bool passed = true;
-        float z = 0.0f;
-        if(has_depth && (op != 7u || (f & VA_F_DWRITE)))
+      auto depth_loop = [&](auto sm) {
+        for(va_cmask todo = cov; todo; todo &= todo - 1u)
         {
-          va_cov_depth(s_rec + k * LS, px, py, &z);
-          if(op != 7u)
-            passed = va_depth_pass(op, z, dep);
-        }
-        if(passed)
-        {
-          pass |= (va_cmask)1u << k;
-          n_pass++;
-          n_blend += (f >> 4) & 1u;
-          if((f & VA_F_DWRITE) && has_depth)
+          const u32 k = va_cmask_ctz(todo);
+          const u32 f = s_rec[k * LS + VA_R_DRAW] >> VA_DRAW_BITS;
+          const u32 op = f & 7u;
+          n_cov++;
+          bool passed = true;
+          float z = 0.0f;
+          if(has_depth && (op != 7u || (f & VA_F_DWRITE)))
           {
-            dep = z;
-            n_dw++;
+            va_cov_depth<decltype(sm)::value>(s_rec + k * LS, px, py, &z);
+            if(op != 7u)
+              passed = va_depth_pass(op, z, dep);
+          }
+          if(passed)
+          {
+            pass |= (va_cmask)1u << k;
+            n_pass++;
+            n_blend += (f >> 4) & 1u;
+            if((f & VA_F_DWRITE) && has_depth)
+            {
+              dep = z;
+              n_dw++;
+            }
           }
         }
-      }
+      };
+      if(chunk_small)
+        depth_loop(va_bool<true>());
+      else
+        depth_loop(va_bool<false>());
       mk[(u64)c * (NW * 64u)] = pass;
       const va_cmask opaque = pass & ~blends;
       if(opaque)
@@ -944,6 +997,7 @@ VA_INLINE void va_raster_tile(const VaDraw *__restrict__ draws, const VaFrame *_
       stg.load_tris(bins, beg, chunk_len(c + 2u), c + 2u);
       if(c + 1u < nchunks)
         m_next = mk[(u64)(c + 1u) * (NW * 64u)];
+      const bool chunk_small = W < (1u << 21) && H < (1u << 21) && __ballot(lane < n && !va_rec_small(s_rec + lane * LS)) == 0ull;
       // this pixel's fragments to shade in this chunk
       const va_cmask alive = cut_c == 0xffffffffu || c > cut_c ? m
                              : c == cut_c                     ? m & (va_cmask)(~(va_cmask)0 << cut_k)
@@ -977,11 +1031,22 @@ VA_INLINE void va_raster_tile(const VaDraw *__restrict__ draws, const VaFrame *_
           const u32 *r = s_rec + k * LS;
           const i32 x = bx + (i32)(p & 7u), y = by + (i32)(p >> 3);
           float b[3][4];
-          va_bary(r, x, y, b[0], nullptr);
+          if(chunk_small)
+          {
+            va_bary<true>(r, x, y, b[0], nullptr);
 #if VA_FS_DERIV
-          va_bary(r, x + 1, y, b[1], nullptr);
-          va_bary(r, x, y + 1, b[2], nullptr);
+            va_bary<true>(r, x + 1, y, b[1], nullptr);
+            va_bary<true>(r, x, y + 1, b[2], nullptr);
 #endif
+          }
+          else
+          {
+            va_bary(r, x, y, b[0], nullptr);
+#if VA_FS_DERIV
+            va_bary(r, x + 1, y, b[1], nullptr);
+            va_bary(r, x, y + 1, b[2], nullptr);
+#endif
+          }
           const VaDraw &d = draws[r[VA_R_DRAW] & VA_DRAW_MASK];
           float o[4];
           va_fs(d.slot, d, (const float *)(r + VA_REC_HDR), b, o);
