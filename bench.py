@@ -205,13 +205,13 @@ def serve(ctx, args):
     from visor_amd import serving
     t0 = time.perf_counter()
     world = serving.ServingWorld(ctx, nclients=args.serve_clients, seed=4)
-    srv = serving.Server(ctx, hbm_budget_bytes=16 << 30, slots=2, max_batch=64)
+    srv = serving.Server(ctx, hbm_budget_bytes=16 << 30, slots=args.serve_slots, max_batch=64)
     for c in world.clients:  # warm: every client once
         srv.wait(srv.submit(c.cb))
     print("[bench] serving world: %d clients, %d-%d triangles per request, built in %.1f s" % (
         len(world.clients), min(c.triangles for c in world.clients), max(c.triangles for c in world.clients),
         time.perf_counter() - t0), file=sys.stderr, flush=True)
-    out = {"clients": len(world.clients), "slots": 2, "hbm_budget_gib": 16, "seconds_per_rate": args.serve_seconds,
+    out = {"clients": len(world.clients), "slots": args.serve_slots, "hbm_budget_gib": 16, "seconds_per_rate": args.serve_seconds,
            "workload": "targets 256^2/512^2/1024^2/1920x1080 BGRA8+D32, 1-32 indexed draws of resident UV-sphere "
                        "meshes (log-uniform total triangles, 1k-500k), Lambert x mipmapped texture 256^2-2048^2, "
                        "bilinear clamp, back-face cull", "rates": []}
@@ -257,6 +257,7 @@ def main():
                          "capacity makes every client resubmit as soon as its frame is done")
     ap.add_argument("--serve-seconds", type=float, default=3.0)
     ap.add_argument("--serve-clients", type=int, default=64)
+    ap.add_argument("--serve-slots", type=int, default=2, help="batches on the GPU at once (1..4)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: every box here has one MI355X; --gpus %d is not supported" % args.gpus,

@@ -16,6 +16,7 @@
 #define VA_MAX_UBO 4
 #define VA_MAX_LEVELS 16
 #define VA_REC_HDR 17   /* header words of a triangle record, varyings follow */
+#define VA_REC_VIS 14   /* leading header words the raster's visibility pass reads */
 #define VA_SETUP_PRIMS 256u /* primitives (threads) per setup workgroup (512: no faster) */
 
 /* record header word indices */
@@ -28,10 +29,10 @@
 #define VA_R_AREA 6
 #define VA_R_INVAREA 7
 #define VA_R_Z0 8
-#define VA_R_IW0 11
-#define VA_R_BBMIN 14
-#define VA_R_BBMAX 15
-#define VA_R_DRAW 16
+#define VA_R_BBMIN 11
+#define VA_R_BBMAX 12
+#define VA_R_DRAW 13
+#define VA_R_IW0 14     /* 1/w of the vertices: only perspective-correct shading reads them */
 
 /* VaDraw.flags */
 #define VA_DRAW_INDEXED 1u

@@ -395,26 +395,14 @@ VA_INLINE u32 va_unorm_out(float v)
 VA_INLINE u32 va_half_bits(float f) { _Float16 h = (_Float16)f; return (u32)__builtin_bit_cast(u16, h); }
 VA_INLINE float va_half_val(u32 b) { return (float)__builtin_bit_cast(_Float16, (u16)b); }
 
-// Edge-function products, modulo 2^32 as the reference's int32 arithmetic wraps. With S (every
-// record of the chunk has window coordinates within +-8191, see va_rec_small, and the target is at
-// most 8192 pixels per side) each factor fits in 24 signed bits and each product is below 2^28, so
-// the 24-bit multiply (full rate, where the 32-bit one issues at a quarter of the rate) gives the
-// same value.
-template <bool S>
-VA_INLINE u32 va_emul(u32 a, u32 b)
-{
-  return S ? (u32)__mul24((int)a, (int)b) : a * b;
-}
-
 // Perspective-correct barycentrics at integer corner (x, y) from the record
 // (rasterizer.cpp:282-310, 551-588). Returns coverage; writes the pre-perspective depth.
-template <bool S = false>
 VA_INLINE bool va_bary(const u32 *__restrict__ r, i32 x, i32 y, float b[4], float *depth)
 {
   const i32 PAx = (i32)(r[VA_R_X0] - (u32)x);
   const i32 PAy = (i32)(r[VA_R_Y0] - (u32)y);
-  const u32 ux = va_emul<S>(r[VA_R_ACX], (u32)PAy) - va_emul<S>(r[VA_R_ACY], (u32)PAx);
-  const u32 uy = va_emul<S>((u32)PAx, r[VA_R_ABY]) - va_emul<S>((u32)PAy, r[VA_R_ABX]);
+  const u32 ux = r[VA_R_ACX] * (u32)PAy - r[VA_R_ACY] * (u32)PAx;
+  const u32 uy = (u32)PAx * r[VA_R_ABY] - (u32)PAy * r[VA_R_ABX];
   const i32 b0 = (i32)(r[VA_R_AREA] - (ux + uy)), b1 = (i32)ux, b2 = (i32)uy;
   const bool covered = b0 >= 0 && b1 >= 0 && b2 >= 0;
   const float invarea = va_u2f(r[VA_R_INVAREA]);
@@ -444,38 +432,36 @@ VA_INLINE float va_factor_value(u32 code, float a)
 // registers. 8-bit targets: dst = (byte[2], byte[1], byte[0]) / 255 with dst alpha 1, factors
 // from the colour factors, bytes B,G,R truncated, alpha byte untouched. RGBA16F targets (beyond
 // the reference) blend all four channels and store halves.
+// The blended values are computed for every fragment and selected (lanes of one step blend
+// fragments of different draws, so a branch would diverge); without blending the fragment's own
+// values are stored, as the reference does (not 1 * src + 0 * dst, which differs for inf / NaN).
 VA_INLINE void va_blend_store_flags(u32 f, bool hdr, float o[4], u32 &c0, u32 &c1)
 {
   const bool blend = (f & VA_F_BLEND) != 0u;
+  const float sf = va_factor_value((f >> 5) & 3u, o[3]);
+  const float df_ = va_factor_value((f >> 7) & 3u, o[3]);
   if(hdr)
   {
-    if(blend)
-    {
-      const float sf = va_factor_value((f >> 5) & 3u, o[3]);
-      const float df_ = va_factor_value((f >> 7) & 3u, o[3]);
-      const float e0 = va_half_val(c0 & 0xffffu), e1 = va_half_val(c0 >> 16);
-      const float e2 = va_half_val(c1 & 0xffffu), e3 = va_half_val(c1 >> 16);
-      o[0] = sf * o[0] + df_ * e0;
-      o[1] = sf * o[1] + df_ * e1;
-      o[2] = sf * o[2] + df_ * e2;
-      o[3] = sf * o[3] + df_ * e3;
-    }
+    const float e0 = va_half_val(c0 & 0xffffu), e1 = va_half_val(c0 >> 16);
+    const float e2 = va_half_val(c1 & 0xffffu), e3 = va_half_val(c1 >> 16);
+    const float b0 = sf * o[0] + df_ * e0, b1 = sf * o[1] + df_ * e1;
+    const float b2 = sf * o[2] + df_ * e2, b3 = sf * o[3] + df_ * e3;
+    o[0] = blend ? b0 : o[0];
+    o[1] = blend ? b1 : o[1];
+    o[2] = blend ? b2 : o[2];
+    o[3] = blend ? b3 : o[3];
     c0 = va_half_bits(o[0]) | (va_half_bits(o[1]) << 16);
     c1 = va_half_bits(o[2]) | (va_half_bits(o[3]) << 16);
   }
   else
   {
-    if(blend)
-    {
-      const float sf = va_factor_value((f >> 5) & 3u, o[3]);
-      const float df_ = va_factor_value((f >> 7) & 3u, o[3]);
-      const float e0 = va_unorm_in((c0 >> 16) & 0xffu);
-      const float e1 = va_unorm_in((c0 >> 8) & 0xffu);
-      const float e2 = va_unorm_in(c0 & 0xffu);
-      o[0] = sf * o[0] + df_ * e0;
-      o[1] = sf * o[1] + df_ * e1;
-      o[2] = sf * o[2] + df_ * e2;
-    }
+    const float e0 = va_unorm_in((c0 >> 16) & 0xffu);
+    const float e1 = va_unorm_in((c0 >> 8) & 0xffu);
+    const float e2 = va_unorm_in(c0 & 0xffu);
+    const float b0 = sf * o[0] + df_ * e0, b1 = sf * o[1] + df_ * e1, b2 = sf * o[2] + df_ * e2;
+    o[0] = blend ? b0 : o[0];
+    o[1] = blend ? b1 : o[1];
+    o[2] = blend ? b2 : o[2];
     c0 = (c0 & 0xff000000u) | (va_unorm_out(o[0]) << 16) | (va_unorm_out(o[1]) << 8) | va_unorm_out(o[2]);
   }
 }
@@ -486,19 +472,14 @@ VA_INLINE void va_blend_store(const VaDraw &d, bool hdr, float o[4], u32 &c0, u3
 }
 
 // depth compare (rasterizer.cpp:562-576)
+// Without branches: VkCompareOp's bits 0, 1 and 2 select less, equal and greater (NEVER 0 ...
+// ALWAYS 7), and an unordered pair (a NaN) passes NOT_EQUAL and ALWAYS only. The same results as
+// the reference's switch (NEVER false, LESS <, EQUAL ==, LEQUAL <=, GREATER >, NOT_EQUAL !=,
+// GEQUAL >=, ALWAYS true); a switch over a per-fragment op compiles to a divergent branch tree.
 VA_INLINE bool va_depth_pass(u32 op, float z, float cur)
 {
-  switch(op)
-  {
-    case 0u: return false;
-    case 1u: return z < cur;
-    case 2u: return z == cur;
-    case 3u: return z <= cur;
-    case 4u: return z > cur;
-    case 5u: return z != cur;
-    case 6u: return z >= cur;
-    default: return true;
-  }
+  const u32 rel = (z < cur ? 1u : 0u) | (z == cur ? 2u : 0u) | (z > cur ? 4u : 0u);
+  return (op & rel) != 0u || (rel == 0u && (op == 5u || op == 7u));
 }
 
 // Orders this wave's LDS writes before its later LDS reads by other lanes.
@@ -585,22 +566,6 @@ struct VaStager
   }
 };
 
-// True when the 24-bit multiply of va_emul is exact for this record at pixel coordinates below
-// 2^22: |X0|, |Y0| < 2^22 and edge coefficients within 24 signed bits keep every factor of the
-// edge products (the coefficients and X0 - x, Y0 - y) within 24 signed bits.
-VA_INLINE bool va_rec_small(const u32 *__restrict__ r)
-{
-  const u32 h22 = 1u << 22, h23 = 1u << 23;
-  return r[VA_R_X0] + h22 < 2u * h22 && r[VA_R_Y0] + h22 < 2u * h22 && r[VA_R_ABX] + h23 < 2u * h23 &&
-         r[VA_R_ABY] + h23 < 2u * h23 && r[VA_R_ACX] + h23 < 2u * h23 && r[VA_R_ACY] + h23 < 2u * h23;
-}
-
-template <bool B>
-struct va_bool
-{
-  static constexpr bool value = B;
-};
-
 // Transposes the 64x64 bit matrix held one row per lane (bit j of lane i = element (i, j)):
 // afterwards bit i of lane j is the former element (i, j). Six block-swap steps, each exchanging
 // the off-diagonal s x s blocks of every 2s x 2s block with the partner lane (lane ^ s).
@@ -655,13 +620,12 @@ VA_INLINE u32 va_wave_exclusive_sum(u32 v, u32 lane)
 }
 
 // Coverage only (the integer part of va_bary).
-template <bool S = false>
 VA_INLINE bool va_covered(const u32 *__restrict__ r, i32 x, i32 y)
 {
   const i32 PAx = (i32)(r[VA_R_X0] - (u32)x);
   const i32 PAy = (i32)(r[VA_R_Y0] - (u32)y);
-  const u32 ux = va_emul<S>(r[VA_R_ACX], (u32)PAy) - va_emul<S>(r[VA_R_ACY], (u32)PAx);
-  const u32 uy = va_emul<S>((u32)PAx, r[VA_R_ABY]) - va_emul<S>((u32)PAy, r[VA_R_ABX]);
+  const u32 ux = r[VA_R_ACX] * (u32)PAy - r[VA_R_ACY] * (u32)PAx;
+  const u32 uy = (u32)PAx * r[VA_R_ABY] - (u32)PAy * r[VA_R_ABX];
   return (i32)(r[VA_R_AREA] - (ux + uy)) >= 0 && (i32)ux >= 0 && (i32)uy >= 0;
 }
 
@@ -676,13 +640,12 @@ VA_INLINE bool va_covered(const u32 *__restrict__ r, i32 x, i32 y)
 // of every fragment are those of the reference; only the lane that runs a fragment program
 // changes.
 // Coverage and interpolated depth only (the first half of va_bary, same operations).
-template <bool S = false>
 VA_INLINE bool va_cov_depth(const u32 *__restrict__ r, i32 x, i32 y, float *depth)
 {
   const i32 PAx = (i32)(r[VA_R_X0] - (u32)x);
   const i32 PAy = (i32)(r[VA_R_Y0] - (u32)y);
-  const u32 ux = va_emul<S>(r[VA_R_ACX], (u32)PAy) - va_emul<S>(r[VA_R_ACY], (u32)PAx);
-  const u32 uy = va_emul<S>((u32)PAx, r[VA_R_ABY]) - va_emul<S>((u32)PAy, r[VA_R_ABX]);
+  const u32 ux = r[VA_R_ACX] * (u32)PAy - r[VA_R_ACY] * (u32)PAx;
+  const u32 uy = (u32)PAx * r[VA_R_ABY] - (u32)PAy * r[VA_R_ABX];
   const i32 b0 = (i32)(r[VA_R_AREA] - (ux + uy)), b1 = (i32)ux, b2 = (i32)uy;
   const float invarea = va_u2f(r[VA_R_INVAREA]);
   float n0 = (float)b0, n1 = (float)b1, n2 = (float)b2;
@@ -819,8 +782,8 @@ VA_INLINE void va_raster_tile(const VaDraw *__restrict__ draws, const VaFrame *_
   va_cmask *mk = masks + (u64)(NW * 64u) * (beg / VA_CHUNK + tile) + wave * 64u + lane;
   u32 cut_c = 0xffffffffu, cut_k = 0u, first_c = 0xffffffffu;
   {
-    // the header words of each record suffice for coverage and depth
-    constexpr u32 HQ = (VA_REC_HDR + 3u) / 4u < Q ? (VA_REC_HDR + 3u) / 4u : Q;
+    // the leading header words of each record (all but 1/w) suffice for coverage and depth
+    constexpr u32 HQ = (VA_REC_VIS + 3u) / 4u < Q ? (VA_REC_VIS + 3u) / 4u : Q;
     VaStager<HQ, NT> stg;
     stg.load_tris(bins, beg, chunk_len(0), 0u);
     stg.load_recs(rec, chunk_len(0));
@@ -840,13 +803,12 @@ VA_INLINE void va_raster_tile(const VaDraw *__restrict__ draws, const VaFrame *_
       //    triangle. Either way lane k ends up holding row k of the coverage matrix
       //    (bit p = pixel p covered by triangle k).
       u64 row_mask = 0ull;
-      bool big = false, wide = false;
+      bool big = false;
       u32 fl = 0u;
       if(lane < n)
       {
         const u32 *r = s_rec + lane * LS;
         fl = r[VA_R_DRAW] >> VA_DRAW_BITS;
-        wide = !va_rec_small(r);
         const u32 bmin = r[VA_R_BBMIN], bmax = r[VA_R_BBMAX];
         const i32 x0 = max((i32)(bmin & 0xffffu), bx), y0 = max((i32)(bmin >> 16), by);
         const i32 x1 = min(min((i32)(bmax & 0xffffu), bx + 8), (i32)W);
@@ -881,25 +843,17 @@ VA_INLINE void va_raster_tile(const VaDraw *__restrict__ draws, const VaFrame *_
       }
       // records of draws that blend (bit k = record k)
       const va_cmask blends = (va_cmask)__ballot((fl & VA_F_BLEND) != 0u);
-      // every record of the chunk allows the 24-bit edge products (a uniform branch below)
-      const bool chunk_small = W < (1u << 21) && H < (1u << 21) && __ballot(wide) == 0ull;
-      auto big_tests = [&](auto sm) {
-        for(u64 bigs = __ballot(big); bigs; bigs &= bigs - 1ull)
-        {
-          const u32 k = (u32)__builtin_ctzll(bigs);
-          const u32 *r = s_rec + k * LS;
-          const u32 bmin = r[VA_R_BBMIN], bmax = r[VA_R_BBMAX];
-          const bool cov = inside && va_covered<decltype(sm)::value>(r, px, py) && px >= (i32)(bmin & 0xffffu) &&
-                           px < (i32)(bmax & 0xffffu) && py >= (i32)(bmin >> 16) && py < (i32)(bmax >> 16);
-          const u64 m = __ballot(cov);
-          if(lane == k)
-            row_mask = m;
-        }
-      };
-      if(chunk_small)
-        big_tests(va_bool<true>());
-      else
-        big_tests(va_bool<false>());
+      for(u64 bigs = __ballot(big); bigs; bigs &= bigs - 1ull)
+      {
+        const u32 k = (u32)__builtin_ctzll(bigs);
+        const u32 *r = s_rec + k * LS;
+        const u32 bmin = r[VA_R_BBMIN], bmax = r[VA_R_BBMAX];
+        const bool cov = inside && va_covered(r, px, py) && px >= (i32)(bmin & 0xffffu) && px < (i32)(bmax & 0xffffu) &&
+                         py >= (i32)(bmin >> 16) && py < (i32)(bmax >> 16);
+        const u64 m = __ballot(cov);
+        if(lane == k)
+          row_mask = m;
+      }
       // 2. Transpose: lane p now holds the set of records (bit k) covering pixel p, so the
       //    in-order loop below runs once per fragment of this pixel, not once per triangle.
 #if VA_CHUNK <= 32
@@ -910,38 +864,25 @@ VA_INLINE void va_raster_tile(const VaDraw *__restrict__ draws, const VaFrame *_
 
       // 3. Early depth test and depth write per pixel in primitive order (ascending k).
       va_cmask pass = 0;
-      auto depth_loop = [&](auto sm) {
-        for(va_cmask todo = cov; todo; todo &= todo - 1u)
-        {
-          const u32 k = va_cmask_ctz(todo);
-          const u32 f = s_rec[k * LS + VA_R_DRAW] >> VA_DRAW_BITS;
-          const u32 op = f & 7u;
-          n_cov++;
-          bool passed = true;
-          float z = 0.0f;
-          if(has_depth && (op != 7u || (f & VA_F_DWRITE)))
-          {
-            va_cov_depth<decltype(sm)::value>(s_rec + k * LS, px, py, &z);
-            if(op != 7u)
-              passed = va_depth_pass(op, z, dep);
-          }
-          if(passed)
-          {
-            pass |= (va_cmask)1u << k;
-            n_pass++;
-            n_blend += (f >> 4) & 1u;
-            if((f & VA_F_DWRITE) && has_depth)
-            {
-              dep = z;
-              n_dw++;
-            }
-          }
-        }
-      };
-      if(chunk_small)
-        depth_loop(va_bool<true>());
-      else
-        depth_loop(va_bool<false>());
+      for(va_cmask todo = cov; todo; todo &= todo - 1u)
+      {
+        const u32 k = va_cmask_ctz(todo);
+        const u32 f = s_rec[k * LS + VA_R_DRAW] >> VA_DRAW_BITS;
+        const u32 op = f & 7u;
+        n_cov++;
+        // straight-line code: the depth of every fragment is computed (ALWAYS without depth
+        // writes is rare), and the test and its consequences are selects, not branches
+        float z = 0.0f;
+        if(has_depth)
+          va_cov_depth(s_rec + k * LS, px, py, &z);
+        const bool passed = !has_depth || va_depth_pass(op, z, dep);
+        const bool dw = passed && has_depth && (f & VA_F_DWRITE) != 0u;
+        pass |= (va_cmask)(passed ? 1u : 0u) << k;
+        n_pass += passed ? 1u : 0u;
+        n_blend += passed ? (f >> 4) & 1u : 0u;
+        n_dw += dw ? 1u : 0u;
+        dep = dw ? z : dep;
+      }
       mk[(u64)c * (NW * 64u)] = pass;
       const va_cmask opaque = pass & ~blends;
       if(opaque)
@@ -997,7 +938,6 @@ VA_INLINE void va_raster_tile(const VaDraw *__restrict__ draws, const VaFrame *_
       stg.load_tris(bins, beg, chunk_len(c + 2u), c + 2u);
       if(c + 1u < nchunks)
         m_next = mk[(u64)(c + 1u) * (NW * 64u)];
-      const bool chunk_small = W < (1u << 21) && H < (1u << 21) && __ballot(lane < n && !va_rec_small(s_rec + lane * LS)) == 0ull;
       // this pixel's fragments to shade in this chunk
       const va_cmask alive = cut_c == 0xffffffffu || c > cut_c ? m
                              : c == cut_c                     ? m & (va_cmask)(~(va_cmask)0 << cut_k)
@@ -1031,22 +971,11 @@ VA_INLINE void va_raster_tile(const VaDraw *__restrict__ draws, const VaFrame *_
           const u32 *r = s_rec + k * LS;
           const i32 x = bx + (i32)(p & 7u), y = by + (i32)(p >> 3);
           float b[3][4];
-          if(chunk_small)
-          {
-            va_bary<true>(r, x, y, b[0], nullptr);
+          va_bary(r, x, y, b[0], nullptr);
 #if VA_FS_DERIV
-            va_bary<true>(r, x + 1, y, b[1], nullptr);
-            va_bary<true>(r, x, y + 1, b[2], nullptr);
+          va_bary(r, x + 1, y, b[1], nullptr);
+          va_bary(r, x, y + 1, b[2], nullptr);
 #endif
-          }
-          else
-          {
-            va_bary(r, x, y, b[0], nullptr);
-#if VA_FS_DERIV
-            va_bary(r, x + 1, y, b[1], nullptr);
-            va_bary(r, x, y + 1, b[2], nullptr);
-#endif
-          }
           const VaDraw &d = draws[r[VA_R_DRAW] & VA_DRAW_MASK];
           float o[4];
           va_fs(d.slot, d, (const float *)(r + VA_REC_HDR), b, o);
