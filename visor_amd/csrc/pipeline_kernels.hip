@@ -100,8 +100,8 @@ VA_INLINE u32 va_index(const VaDraw &d, u32 i)
 {
   if(!(d.flags & VA_DRAW_INDEXED)) return i;
   // an index past the end of the bound index buffer reads as 0 instead of faulting
-  if(d.flags & VA_DRAW_INDEX32) return 4ull * i + 4ull <= d.ib_bytes ? ((const u32 *)d.ib)[i] : 0u;
-  return 2ull * i + 2ull <= d.ib_bytes ? ((const u16 *)d.ib)[i] : 0u;
+  if(d.flags & VA_DRAW_INDEX32) return 4ull * i + 4ull <= d.ib_bytes ? va_gld((const u32 *)d.ib + i) : 0u;
+  return 2ull * i + 2ull <= d.ib_bytes ? *((const VA_GLOBAL u16 *)d.ib + i) : 0u;
 }
 
 // True when the raster's 32-bit edge arithmetic cannot wrap for this triangle: window
@@ -713,13 +713,13 @@ VA_INLINE void va_raster_tile(const VaDraw *__restrict__ draws, const VaFrame *_
     else if(hdr)
     {
       const u32 *cp = (const u32 *)fr.color + pix * 2u;
-      c0 = cp[0];
-      c1 = cp[1];
+      c0 = va_gld(cp);
+      c1 = va_gld(cp + 1);
     }
     else
-      c0 = ((const u32 *)fr.color)[pix];
+      c0 = va_gld((const u32 *)fr.color + pix);
     if(has_depth)
-      dep = (fused && (fr.clear_flags & VA_CLEAR_DEPTH)) ? fr.clear_depth : ((const float *)fr.depth)[pix];
+      dep = (fused && (fr.clear_flags & VA_CLEAR_DEPTH)) ? fr.clear_depth : va_u2f(va_gld((const u32 *)fr.depth + pix));
   }
 
   u32 n_cov = 0, n_pass = 0, n_blend = 0, n_dw = 0;
@@ -1002,13 +1002,13 @@ VA_INLINE void va_raster_tile(const VaDraw *__restrict__ draws, const VaFrame *_
     if(hdr)
     {
       u32 *cp = (u32 *)fr.color + pix * 2u;
-      cp[0] = c0;
-      cp[1] = c1;
+      va_gst(cp, c0);
+      va_gst(cp + 1, c1);
     }
     else
-      ((u32 *)fr.color)[pix] = c0;
+      va_gst((u32 *)fr.color + pix, c0);
     if(has_depth)
-      ((float *)fr.depth)[pix] = dep;
+      va_gst((u32 *)fr.depth + pix, va_f2u(dep));
   }
   // columns in VaCounters order from pixels_written
   va_block_partials(row, n_cov, n_pass, n_blend, n_dw);

@@ -13,6 +13,19 @@ typedef unsigned char u8;
 
 #define VA_INLINE __device__ __forceinline__
 
+// Loads and stores of buffer, texture and target memory through global-address-space pointers.
+// These pointers come out of the draw and frame tables as integers, so the compiler would use
+// flat instructions, which may also address LDS and therefore also count on the LDS wait counter
+// (every wait for a texel then waits for outstanding LDS operations too). Every such buffer is
+// device memory (host-visible memory is mirrored to the device around a submission).
+#ifdef __AMDGCN__
+#define VA_GLOBAL __attribute__((address_space(1)))
+#else
+#define VA_GLOBAL
+#endif
+VA_INLINE u32 va_gld(const void *p) { return *(const VA_GLOBAL u32 *)p; }
+VA_INLINE void va_gst(void *p, u32 v) { *(VA_GLOBAL u32 *)p = v; }
+
 VA_INLINE float va_u2f(u32 u) { return __uint_as_float(u); }
 VA_INLINE u32 va_f2u(float f) { return __float_as_uint(f); }
 // float -> int32 as the reference's x64 build converts (cvttss2si): truncation inside
@@ -95,7 +108,7 @@ VA_INLINE u32 va_pc_u(const VaDraw &d, u32 off) { return off <= 124u ? *(const u
 VA_INLINE float va_pc_f(const VaDraw &d, u32 off) { return va_u2f(va_pc_u(d, off)); }
 VA_INLINE u32 va_ubo_u(const VaDraw &d, u32 slot, u64 off)
 {
-  return off + 4ull <= d.ubo_bytes[slot] ? *(const u32 *)(d.ubo[slot] + off) : 0u;
+  return off + 4ull <= d.ubo_bytes[slot] ? va_gld((const void *)(d.ubo[slot] + off)) : 0u;
 }
 VA_INLINE float va_ubo_f(const VaDraw &d, u32 slot, u64 off) { return va_u2f(va_ubo_u(d, slot, off)); }
 
@@ -113,10 +126,10 @@ VA_INLINE void va_vattr(const VaDraw &d, u32 vtx, u32 loc, float out[4])
   // dropped, so the loads need no branch and can be issued together with the others
   const bool ok = d.vb[vb] != 0ull && off + need <= d.vb_bytes[vb];
   const u32 *w = ok ? (const u32 *)((const u8 *)d.vb[vb] + off) : (const u32 *)d.push;
-  const u32 w0 = w[0];
-  const u32 w1 = need >= 8u ? w[1] : 0u;
-  const u32 w2 = need >= 12u ? w[2] : 0u;
-  const u32 w3 = need >= 16u ? w[3] : 0u;
+  const u32 w0 = va_gld(w);
+  const u32 w1 = need >= 8u ? va_gld(w + 1) : 0u;
+  const u32 w2 = need >= 12u ? va_gld(w + 2) : 0u;
+  const u32 w3 = need >= 16u ? va_gld(w + 3) : 0u;
   float o[4] = {0.0f, 0.0f, 0.0f, 1.0f};
   switch(fmt)
   {
@@ -145,7 +158,7 @@ VA_INLINE void va_vattr(const VaDraw &d, u32 vtx, u32 loc, float out[4])
 
 VA_INLINE void va_texel(const u8 *base, u32 W, i32 x, i32 y, float out[4])
 {
-  u32 u = *(const u32 *)(base + ((u64)(u32)y * W + (u32)x) * 4u);
+  u32 u = va_gld(base + ((u64)(u32)y * W + (u32)x) * 4u);
   out[0] = va_unorm8(u & 0xffu);
   out[1] = va_unorm8((u >> 8) & 0xffu);
   out[2] = va_unorm8((u >> 16) & 0xffu);
