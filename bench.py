@@ -144,6 +144,15 @@ def measure(ctx, scene, texel_bytes, steps, warmup, plain_steps, tile=0, graphs=
            "graph_replays": acc["graph_launches"], "algorithmic_bytes": b_raster,
            "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                         "frac": round(achieved / HBM_PEAK_GBS, 4)}}
+    # back-to-back submissions (asynchronous submit, one wait at the end): the rate when the host
+    # queues the next frame while the GPU renders the current one
+    ctx.set_mode(batched=True, async_submit=True)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        ctx.icd.check(ctx.submit([r.cb]), "vkQueueSubmit")
+    ctx.icd.check(ctx.icd.vkQueueWaitIdle(ctx.queue), "vkQueueWaitIdle")
+    out["pipelined_gfragments_s"] = frags * steps / (time.perf_counter() - t0) / 1e9
+    ctx.set_mode(batched=True, async_submit=False)
     if plain_steps:
         ctx.set_mode(batched=False)
         r.render()
@@ -238,6 +247,31 @@ def serve(ctx, args):
     return out
 
 
+def transfer(ctx, gib, reps=3):
+    """vkCmdCopyBuffer between two device-local buffers of `gib` GiB (the transfer path,
+    cmd_exec.cpp:174-182): one submission per copy; also the known-size copy that calibrates the
+    HBM byte counters of the profiles."""
+    from visor_amd import vk
+    import ctypes as C
+    n = int(gib * (1 << 30))
+    src, smem = ctx.buffer(n, host_visible=False)
+    dst, dmem = ctx.buffer(n, host_visible=False)
+    region = vk.BufferCopy(srcOffset=0, dstOffset=0, size=n)
+    cb = ctx.begin()
+    ctx.icd.vkCmdCopyBuffer(cb, src, dst, 1, C.byref(region))
+    ctx.icd.check(ctx.icd.vkEndCommandBuffer(cb), "vkEndCommandBuffer")
+    times = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        ctx.icd.check(ctx.submit([cb]), "vkQueueSubmit")
+        ctx.icd.check(ctx.icd.vkQueueWaitIdle(ctx.queue), "vkQueueWaitIdle")
+        times.append(time.perf_counter() - t0)
+    ctx.free_buffer(src, smem)
+    ctx.free_buffer(dst, dmem)
+    best = min(times)
+    return {"bytes_copied": n, "best_ms": round(best * 1e3, 3), "gb_s_read_plus_write": round(2 * n / best / 1e9, 1)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -258,6 +292,8 @@ def main():
     ap.add_argument("--serve-seconds", type=float, default=3.0)
     ap.add_argument("--serve-clients", type=int, default=64)
     ap.add_argument("--serve-slots", type=int, default=2, help="batches on the GPU at once (1..4)")
+    ap.add_argument("--transfer-gib", type=float, default=0.0,
+                    help="only time a device-local vkCmdCopyBuffer of this many GiB (three copies) and exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: every box here has one MI355X; --gpus %d is not supported" % args.gpus,
@@ -268,6 +304,9 @@ def main():
         os.environ["VISOR_CHUNK"] = str(args.chunk)  # read when the queue is created
     from visor_amd import render, scenes
 
+    if args.transfer_gib > 0:
+        print(json.dumps({"transfer": transfer(render.Context(), args.transfer_gib)}))
+        return
     scene = scenes.many_draws()
     ctx = render.Context()
     print("[bench] many_draws: %d draws, %d triangles; warmup %d" % (len(scene.draws), scene.triangles, args.warmup),
@@ -318,6 +357,7 @@ def main():
                    "mode": "batched", "tile": tile, "chunk": one["chunk"], "persistent": bool(one["persistent"]),
                    "launch_shape": "autotuned" if args.tile == 0 else "fixed", "tuner_trial_frames": m["trials"],
                    "fragments_per_frame": m["fragments"], "shaded_per_frame": m["shaded"]},
+        "pipelined_gfrag_s": round(m["pipelined_gfragments_s"], 4),
         "plain_per_draw_gfrag_s": round(plain, 4) if plain else None,
         "speedup_vs_plain": round(m["gfragments_s"] / plain, 2) if plain else None,
         "frame_latency_ms": {"p50": round(statistics.median(lat_ms), 3),
