@@ -227,16 +227,29 @@ static std::string ModuleSource(const std::vector<std::shared_ptr<ShaderPair>> &
   s << "#define VA_SMALL " << cfg.small << "\n";
   for(auto &p : pairs)
     s << p->vs.code << "\n" << p->fs.code << "\n";
-  s << "VA_INLINE void va_vs(u32 slot, const VaDraw &d, u32 vtx, float pos[4], float *vary)\n{\n";
-  s << "  float outs[40];\n  switch(slot)\n  {\n";
+  // the vertex program of a primitive's three vertices: the raw words of every attribute the
+  // program reads are loaded for all three vertices first (one basic block, no branch, so the
+  // loads are issued together), then the program runs per vertex on them
+  s << "VA_INLINE void va_vs3(u32 slot, const VaDraw &d, const u32 vtx[3], float pos[3][4], float vary[3][VA_MAXNV > 0 ? "
+       "VA_MAXNV : 1])\n{\n";
+  s << "  float outs[3][40];\n  VaRaw raw[3][16];\n  switch(slot)\n  {\n";
   for(size_t i = 0; i < pairs.size(); i++)
   {
     const auto &p = pairs[i];
-    s << "    case " << i << "u:\n      " << p->vs.fn_name << "(d, vtx, pos, outs);\n";
-    for(uint32_t j = 0; j < p->nv; j++)
-      s << "      vary[" << j << "] = outs[" << p->vary_src[j] << "];\n";
-    for(uint32_t j = p->nv; j < maxnv; j++)
-      s << "      vary[" << j << "] = 0.0f;\n";
+    s << "    case " << i << "u:\n";
+    for(uint32_t loc : p->vs.vs_locations)
+      for(int k = 0; k < 3; k++)
+        s << "      raw[" << k << "][" << loc << "] = va_vload(d, vtx[" << k << "], " << loc << "u);\n";
+    for(int k = 0; k < 3; k++)
+      s << "      " << p->vs.fn_name << "(d, vtx[" << k << "], raw[" << k << "], pos[" << k << "], outs[" << k
+        << "]);\n";
+    for(int k = 0; k < 3; k++)
+    {
+      for(uint32_t j = 0; j < p->nv; j++)
+        s << "      vary[" << k << "][" << j << "] = outs[" << k << "][" << p->vary_src[j] << "];\n";
+      for(uint32_t j = p->nv; j < maxnv; j++)
+        s << "      vary[" << k << "][" << j << "] = 0.0f;\n";
+    }
     s << "      break;\n";
   }
   s << "    default: break;\n  }\n}\n";

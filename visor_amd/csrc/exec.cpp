@@ -1070,7 +1070,9 @@ VkResult Queue::RunPass(const std::vector<Op *> &passes, bool fused_clear, ExecS
             return hipErrorUnknown;
           uint32_t *st_ = (uint32_t *)d_start;
           void *rargs[] = {&sorted_keys, &total_dev, &cap, &tile_total, &st_};
-          VA_TRY(hipModuleLaunchKernel(mod->ranges, (tile_total + 256) / 256, 1, 1, 256, 1, 1, 0, stream, rargs, nullptr));
+          // a grid-stride pass over the (at most cap + 1) bin boundaries
+          const uint32_t rblocks = (uint32_t)std::min<uint64_t>(((uint64_t)cap + 256) / 256, 1024);
+          VA_TRY(hipModuleLaunchKernel(mod->ranges, rblocks, 1, 1, 256, 1, 1, 0, stream, rargs, nullptr));
         }
         else
         {

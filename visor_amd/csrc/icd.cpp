@@ -390,7 +390,9 @@ VKAPI_ATTR VkResult VKAPI_CALL vkAllocateMemory(VkDevice, const VkMemoryAllocate
   VkDeviceMemory m = new VkDeviceMemory_T;
   m->size = info->allocationSize;
   m->type = info->memoryTypeIndex == 1 ? 1 : 0;
-  size_t bytes = std::max<size_t>(m->size, 4);
+  // 16 bytes beyond the allocation stay readable: the vertex fetch loads 16 bytes from the start
+  // of every attribute that lies inside its buffer
+  size_t bytes = std::max<size_t>(m->size, 4) + 16;
   if(hipMalloc((void **)&m->dev, bytes) != hipSuccess)
   {
     delete m;

@@ -2,7 +2,8 @@
 //
 // Included after the translated shaders, which define:
 //   VA_NSLOTS, VA_MAXNV, VA_REC_STRIDE, VA_FS_DERIV
-//   va_vs(slot, draw, vertexIndex, pos[4], vary[VA_MAXNV])   vertex program + varying gather
+//   va_vs3(slot, draw, vertexIndex[3], pos[3][4], vary[3][VA_MAXNV])
+//                                                   vertex program + varying gather, 3 vertices
 //   va_fs(slot, draw, vary, b[3][4], out[4]) -> keep          fragment program
 //
 // Stages (reference rasterizer.cpp:363-696 in GPU form):
@@ -81,6 +82,39 @@ VA_INLINE void va_block_partials(u64 *row, u64 a, u64 b, u64 c, u64 e)
   }
 }
 
+extern "C" __device__ __attribute__((const)) u32 __ockl_wfred_add_u32(u32);
+
+// va_setup's counters over the workgroup as one row (tris_in, tris_out, blocks, pixels_tested):
+// the two flags are counted by ballot, blocks and pixels_tested summed as 32-bit values across
+// the wave (pixels_tested in 16-bit halves), then across the waves in LDS. Exact while a
+// primitive's blocks and tested pixels stay below 2^26 and 2^42, i.e. for targets of at most
+// 65536 pixels per side (the device allows 16384).
+VA_INLINE void va_setup_partials(u64 *row, bool fin, bool fout, u32 blocks, u64 tested)
+{
+  __shared__ u64 s_part[VA_SETUP_PRIMS / 64u][4];
+  const u64 a = (u64)__popcll(__ballot(fin));
+  const u64 b = (u64)__popcll(__ballot(fout));
+  const u64 c = (u64)__ockl_wfred_add_u32(blocks);
+  const u64 e = (u64)__ockl_wfred_add_u32((u32)tested & 0xffffu) + ((u64)__ockl_wfred_add_u32((u32)(tested >> 16)) << 16);
+  const u32 wave = threadIdx.x >> 6;
+  if((threadIdx.x & 63u) == 0u)
+  {
+    s_part[wave][0] = a;
+    s_part[wave][1] = b;
+    s_part[wave][2] = c;
+    s_part[wave][3] = e;
+  }
+  __syncthreads();
+  if(threadIdx.x < 4u)
+  {
+    u64 t = 0;
+#pragma unroll
+    for(u32 w = 0; w < VA_SETUP_PRIMS / 64u; w++)
+      t += s_part[w][threadIdx.x];
+    row[threadIdx.x] = t;
+  }
+}
+
 // Fixed-function state of a record's draw in 9 bits: depth op (0-2), depth write (3), blend
 // enable (4), source and destination colour factor codes (5-6, 7-8: 0 ZERO, 1 ONE and every
 // other factor, 2 SRC_ALPHA, 3 ONE_MINUS_SRC_ALPHA; reference rasterizer.cpp:600-640).
@@ -95,13 +129,38 @@ VA_INLINE u16 va_rec_flags(const VaDraw &d)
   return (u16)((d.depth_op & 7u) | (d.depth_write ? VA_F_DWRITE : 0u) | (d.blend_enable ? VA_F_BLEND : 0u) |
                (va_factor_code(d.src_factor) << 5) | (va_factor_code(d.dst_factor) << 7));
 }
-// reference rasterizer.cpp:100-119 (GetIndex, vertexOffset not applied)
-VA_INLINE u32 va_index(const VaDraw &d, u32 i)
+// reference rasterizer.cpp:100-119 (GetIndex, vertexOffset not applied), for the three vertices
+// of a primitive at once: an index past the end of the bound index buffer reads as 0 instead of
+// faulting. Such an index is loaded from va_zero_words, so every load is unconditional and needs
+// no select afterwards, and the three are issued together.
+VA_INLINE void va_index3(const VaDraw &d, const u32 p[3], u32 v[3])
 {
-  if(!(d.flags & VA_DRAW_INDEXED)) return i;
-  // an index past the end of the bound index buffer reads as 0 instead of faulting
-  if(d.flags & VA_DRAW_INDEX32) return 4ull * i + 4ull <= d.ib_bytes ? va_gld((const u32 *)d.ib + i) : 0u;
-  return 2ull * i + 2ull <= d.ib_bytes ? *((const VA_GLOBAL u16 *)d.ib + i) : 0u;
+  if(!(d.flags & VA_DRAW_INDEXED))
+  {
+#pragma unroll
+    for(int k = 0; k < 3; k++)
+      v[k] = p[k];
+    return;
+  }
+  const VA_GLOBAL u8 *zw = (const VA_GLOBAL u8 *)va_zero_words;
+  const VA_GLOBAL u8 *ib = (const VA_GLOBAL u8 *)d.ib;
+  const u32 sh = (d.flags & VA_DRAW_INDEX32) ? 2u : 1u;
+  const VA_GLOBAL u8 *a[3];
+#pragma unroll
+  for(int k = 0; k < 3; k++)
+    a[k] = ((u64)p[k] << sh) + (1ull << sh) <= d.ib_bytes ? ib + ((u64)p[k] << sh) : zw;
+  if(sh == 2u)
+  {
+#pragma unroll
+    for(int k = 0; k < 3; k++)
+      v[k] = *(const VA_GLOBAL u32 *)a[k];
+  }
+  else
+  {
+#pragma unroll
+    for(int k = 0; k < 3; k++)
+      v[k] = *(const VA_GLOBAL u16 *)a[k];
+  }
 }
 
 // True when the raster's 32-bit edge arithmetic cannot wrap for this triangle: window
@@ -181,7 +240,8 @@ extern "C" __global__ __launch_bounds__(VA_SETUP_PRIMS) void va_setup(
   const VaDraw &d = draws[di];
   const u32 prim = (blockIdx.x - d.block_base) * VA_SETUP_PRIMS + threadIdx.x;
   const u32 g = d.tri_base + prim;
-  u64 n_in = 0, n_out = 0, n_tested = 0, n_blocks = 0;
+  u32 n_in = 0, n_out = 0, n_blocks = 0;
+  u64 n_tested = 0;
   if(prim < d.ntris)
   {
     const VaFrame &fr = frames[d.frame];
@@ -198,9 +258,10 @@ extern "C" __global__ __launch_bounds__(VA_SETUP_PRIMS) void va_setup(
     }
     float pos[3][4];
     float vary[3][VA_MAXNV > 0 ? VA_MAXNV : 1];
-    va_vs(d.slot, d, va_index(d, d.first + p0), pos[0], vary[0]);
-    va_vs(d.slot, d, va_index(d, d.first + p1), pos[1], vary[1]);
-    va_vs(d.slot, d, va_index(d, d.first + p2), pos[2], vary[2]);
+    const u32 pi[3] = {d.first + p0, d.first + p1, d.first + p2};
+    u32 vi[3];
+    va_index3(d, pi, vi);
+    va_vs3(d.slot, d, vi, pos, vary);
 
     // ToWindow (rasterizer.cpp:239-253)
     const float Wf = (float)fr.width, Hf = (float)fr.height;
@@ -238,7 +299,7 @@ extern "C" __global__ __launch_bounds__(VA_SETUP_PRIMS) void va_setup(
         maxy = min((i32)fr.height - 1, maxy);
         // 32x32 job count as the reference computes it (C truncating division), for the counter
         i32 xb = 1 + (maxx - minx) / 32, yb = 1 + (maxy - miny) / 32;
-        n_blocks = (u64)(xb > 0 ? xb : 0) * (u64)(yb > 0 ? yb : 0);
+        n_blocks = (u32)(xb > 0 ? xb : 0) * (u32)(yb > 0 ? yb : 0);
         if(maxx > minx && maxy > miny)
         {
           n_tested = (u64)(maxx - minx) * (u64)(maxy - miny);
@@ -295,7 +356,7 @@ extern "C" __global__ __launch_bounds__(VA_SETUP_PRIMS) void va_setup(
   }
   // counters: one row of partial sums per workgroup (no contended atomics), columns in
   // VaCounters order from tris_in
-  va_block_partials(part + (u64)blockIdx.x * 4u, n_in, n_out, n_blocks, n_tested);
+  va_setup_partials(part + (u64)blockIdx.x * 4u, n_in != 0u, n_out != 0u, n_blocks, n_tested);
 }
 
 // Position of the k-th set bit of m (k < popcount(m)).
@@ -362,19 +423,20 @@ extern "C" __global__ __launch_bounds__(VA_EMIT_G) void va_emit(
 }
 
 // Bin bounds from the sorted keys: start[t] = first index whose key is >= t, for every tile
-// t in [0, ntiles] (so tile t's bin is [start[t], start[t + 1])). n = min(*n_dev, cap).
+// t in [0, ntiles] (so tile t's bin is [start[t], start[t + 1])), n = min(*n_dev, cap). One
+// coalesced pass over the keys: index i is the start of every tile t with
+// key[i - 1] < t <= key[i] (key[-1] = -1, key[n] = ntiles), so each entry is written once.
 extern "C" __global__ __launch_bounds__(256) void va_ranges(const u32 *__restrict__ keys, const u32 *__restrict__ n_dev,
                                                             u32 cap, u32 ntiles, u32 *__restrict__ start)
 {
-  const u32 t = blockIdx.x * blockDim.x + threadIdx.x;
-  if(t > ntiles) return;
-  u32 lo = 0, hi = min(*n_dev, cap);
-  while(lo < hi)
+  const u32 n = min(*n_dev, cap);
+  for(u32 i = blockIdx.x * 256u + threadIdx.x; i <= n; i += gridDim.x * 256u)
   {
-    const u32 mid = (lo + hi) >> 1;
-    if(keys[mid] < t) lo = mid + 1u; else hi = mid;
+    const u32 lo = i == 0u ? 0u : keys[i - 1u] + 1u;
+    const u32 hi = i == n ? ntiles : keys[i];
+    for(u32 t = lo; t <= hi; t++)
+      start[t] = i;
   }
-  start[t] = lo;
 }
 
 extern "C" __global__ __launch_bounds__(256) void va_fill_u32(u32 *__restrict__ p, u32 n, u32 v)

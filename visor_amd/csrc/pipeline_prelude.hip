@@ -112,43 +112,72 @@ VA_INLINE u32 va_ubo_u(const VaDraw &d, u32 slot, u64 off)
 }
 VA_INLINE float va_ubo_f(const VaDraw &d, u32 slot, u64 off) { return va_u2f(va_ubo_u(d, slot, off)); }
 
+// 16 zero bytes: the load address of an index or a vertex attribute outside its bound buffer.
+__device__ u32 va_zero_words[4] __attribute__((aligned(16))) = {0u, 0u, 0u, 0u};  // never written
+
 // GetVertexAttributeData: default (0,0,0,1); 32-bit formats copy raw bits with fall-through,
 // R8G8B8A8_UNORM divides each byte by 255.
-VA_INLINE void va_vattr(const VaDraw &d, u32 vtx, u32 loc, float out[4])
+// In two steps, so that the loads of every attribute of a primitive's three vertices are issued
+// together and waited for once: va_vload has no branch and issues one 16-byte load per attribute
+// (an attribute past the end of its bound buffer loads va_zero_words instead), and va_vdecode
+// turns the raw words into the attribute's four values, dropping the words beyond the format's.
+// Device memory is allocated 16 bytes longer than asked, so the 16 bytes from an attribute that
+// lies inside its buffer are always readable.
+struct VaRaw
+{
+  u32 w[4];
+  bool ok;  // the attribute lies inside its bound buffer
+};
+
+// four words at 4-byte alignment (vertex attributes are aligned to their 4-byte components)
+struct __attribute__((packed, aligned(4))) VaWords4
+{
+  u32 x, y, z, w;
+};
+
+// words of a format: R32 .. R32G32B32A32 in UINT, SINT, SFLOAT (VK_FORMAT 98..109, three formats
+// per width) take 1..4, R8G8B8A8_UNORM (37) and every other format 1
+VA_INLINE u32 va_vfmt_words(u32 fmt) { return fmt >= 98u && fmt <= 109u ? (fmt - 98u) / 3u + 1u : 1u; }
+
+VA_INLINE VaRaw va_vload(const VaDraw &d, u32 vtx, u32 loc)
 {
   const u32 vb = d.vattr_vb[loc];
   const u64 off = d.vattr_off[loc] + (u64)d.vattr_stride[loc] * vtx;
+  const u32 nw = va_vfmt_words(d.vattr_fmt[loc]);
+  VaRaw r;
+  r.ok = (d.vb[vb] != 0ull) & (off + 4ull * nw <= d.vb_bytes[vb]);
+  const VA_GLOBAL u8 *base = (const VA_GLOBAL u8 *)d.vb[vb];
+  const VA_GLOBAL VaWords4 *p = r.ok ? (const VA_GLOBAL VaWords4 *)(base + off) : (const VA_GLOBAL VaWords4 *)va_zero_words;
+  const VaWords4 x = *p;
+  r.w[0] = x.x;
+  r.w[1] = x.y;
+  r.w[2] = x.z;
+  r.w[3] = x.w;
+  return r;
+}
+
+VA_INLINE void va_vdecode(const VaDraw &d, u32 loc, const VaRaw &r, float out[4])
+{
   const u32 fmt = d.vattr_fmt[loc];
-  const u32 need = (fmt == 109u || fmt == 107u || fmt == 108u) ? 16u : (fmt == 106u || fmt == 104u || fmt == 105u) ? 12u
-                   : (fmt == 103u || fmt == 101u || fmt == 102u) ? 8u : 4u;
-  // an attribute past the end of the bound buffer reads the defaults instead of faulting; its
-  // loads go to a readable stand-in (the draw's push-constant bytes) and their values are
-  // dropped, so the loads need no branch and can be issued together with the others
-  const bool ok = d.vb[vb] != 0ull && off + need <= d.vb_bytes[vb];
-  const u32 *w = ok ? (const u32 *)((const u8 *)d.vb[vb] + off) : (const u32 *)d.push;
-  const u32 w0 = va_gld(w);
-  const u32 w1 = need >= 8u ? va_gld(w + 1) : 0u;
-  const u32 w2 = need >= 12u ? va_gld(w + 2) : 0u;
-  const u32 w3 = need >= 16u ? va_gld(w + 3) : 0u;
   float o[4] = {0.0f, 0.0f, 0.0f, 1.0f};
-  switch(fmt)
+  if(fmt >= 98u && fmt <= 109u)
   {
-    case 109: case 107: case 108: o[3] = va_u2f(w3);
-    case 106: case 104: case 105: o[2] = va_u2f(w2);
-    case 103: case 101: case 102: o[1] = va_u2f(w1);
-    case 100: case 98: case 99: o[0] = va_u2f(w0); break;
-    case 37:
-      o[0] = va_unorm8((w0 & 0x000000ffu) >> 0);
-      o[1] = va_unorm8((w0 & 0x0000ff00u) >> 8);
-      o[2] = va_unorm8((w0 & 0x00ff0000u) >> 16);
-      o[3] = va_unorm8((w0 & 0xff000000u) >> 24);
-      break;
-    default: break;
+    const u32 nw = va_vfmt_words(fmt);
+#pragma unroll
+    for(u32 k = 0; k < 4u; k++)
+      if(k < nw)
+        o[k] = va_u2f(r.w[k]);
   }
-  out[0] = ok ? o[0] : 0.0f;
-  out[1] = ok ? o[1] : 0.0f;
-  out[2] = ok ? o[2] : 0.0f;
-  out[3] = ok ? o[3] : 1.0f;
+  else if(fmt == 37u)
+  {
+#pragma unroll
+    for(u32 k = 0; k < 4u; k++)
+      o[k] = va_unorm8((r.w[0] >> (8u * k)) & 0xffu);
+  }
+  out[0] = r.ok ? o[0] : 0.0f;
+  out[1] = r.ok ? o[1] : 0.0f;
+  out[2] = r.ok ? o[2] : 0.0f;
+  out[3] = r.ok ? o[3] : 1.0f;
 }
 
 // ---------------------------------------------------------------------------------------------

@@ -9,6 +9,7 @@
 // df type when the fragment program needs derivatives.
 #include "spirv_hip.h"
 
+#include <algorithm>
 #include <cstdio>
 #include <cstring>
 #include <functional>
@@ -1656,8 +1657,16 @@ bool Translator::Run(const char *entry, std::string &err)
         {
           // reference :1952-1986: GetVertexAttributeData, then the first N components
           uint32_t n = FlatCount(pointee);
+          if(loc[0] >= 16)
+          {
+            err = "vertex input location beyond the 16 vertex attributes";
+            return false;
+          }
+          if(std::find(out.vs_locations.begin(), out.vs_locations.end(), loc[0]) == out.vs_locations.end())
+            out.vs_locations.push_back(loc[0]);
           std::string t = "va" + std::to_string(id);
-          pro << "  { float " << t << "[4]; va_vattr(d, vtx, " << loc[0] << "u, " << t << ");\n";
+          // raw[loc] holds the attribute's words, loaded before the program runs
+          pro << "  { float " << t << "[4]; va_vdecode(d, " << loc[0] << "u, raw[" << loc[0] << "], " << t << ");\n";
           for(uint32_t k = 0; k < n && k < 4; k++)
           {
             Leaf l = LeafKind(pointee, k);
@@ -1797,7 +1806,8 @@ bool Translator::Run(const char *entry, std::string &err)
   std::ostringstream src;
   if(stage == ShaderStage::Vertex)
   {
-    src << "VA_INLINE void " << out.fn_name << "(const VaDraw &d, u32 vtx, float pos[4], float outs[40])\n{\n";
+    src << "VA_INLINE void " << out.fn_name
+        << "(const VaDraw &d, u32 vtx, const VaRaw *raw, float pos[4], float outs[40])\n{\n";
     src << "  typedef float F;\n";
     src << "#pragma unroll\n  for(int i_ = 0; i_ < 40; i_++) outs[i_] = 0.0f;\n";
   }

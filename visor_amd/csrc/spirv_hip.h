@@ -38,6 +38,7 @@ struct TranslatedShader
   std::string fn_name;
   std::string code;
   std::vector<FsInput> fs_inputs;         // fragment inputs in varying order
+  std::vector<uint32_t> vs_locations;     // vertex input locations the vertex program reads
   std::vector<ShaderResource> resources;  // resources referenced
   bool needs_deriv = false;               // implicit-LOD sampling or derivative ops
   bool has_kill = false;
