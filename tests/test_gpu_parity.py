@@ -90,6 +90,38 @@ def test_random_triangles_seeds_bit_exact(ctx, seed):
     assert out["stats"]["pixels_written"] == ref["counters"]["pixels_written"]
 
 
+def _stacked_triangles(ntris, width, height, seed):
+    """ntris triangles that each cover the whole target, depths on 8 levels (LESS: the first
+    triangle of a level wins, so the result depends on the order inside every bin), random
+    colours, alternating winding."""
+    rng = np.random.default_rng(seed)
+    a = np.array([[-1.0, -1.0], [3.0, -1.0], [-1.0, 3.0]], np.float32)
+    b = a[[0, 2, 1]]
+    xy = np.where((np.arange(ntris) % 2 == 0)[:, None, None], a[None], b[None]).reshape(-1, 2)
+    pos = np.zeros((3 * ntris, 4), np.float32)
+    pos[:, :2] = xy
+    pos[:, 2] = np.repeat((rng.integers(0, 8, ntris) / 8.0).astype(np.float32), 3)
+    pos[:, 3] = 1.0
+    col = rng.integers(0, 256, (3 * ntris, 4), dtype=np.uint8)
+    pipes = {"main": scenes.Pipeline("vs_mvp_pos4_col", "fs_color")}
+    draws = [scenes.Draw("main", 3 * ntris, 0, False, scenes.push_block(scenes.IDENTITY))]
+    return scenes.Scene("stacked", width, height, scenes.interleave_pos_col(pos, col), 20, scenes.VATTRS_POS_COL, pipes,
+                        draws)
+
+
+@pytest.mark.parametrize("ntris", [50, 500, 3000, 17000, 70000])
+def test_stacked_bins_keep_api_order(ctx, ntris):
+    """Every tile's bin holds all ntris triangles (bins of 50 to 70000 pairs, 70000 primitives
+    needing three 8-bit digits of the primitive index); the colours equal the oracle's only if
+    every bin is walked in API order."""
+    s = _stacked_triangles(ntris, 24, 16, seed=ntris)
+    ref = refrast.render(s)
+    out = _render_gpu(ctx, s, tile=8)
+    _compare(s, out, ref)
+    assert out["stats"]["pixels_written"] == ref["counters"]["pixels_written"]
+    assert out["stats"]["depth_passed"] == ref["counters"]["depth_passed"]
+
+
 def test_bin_overflow_reruns_the_pass(ctx):
     """2000 large random triangles on 640x480 with 8x8 tiles make about 2.5M (tile, triangle)
     pairs, over twice the first bin capacity of a new queue (max(2^20, 2 x triangles)): the first

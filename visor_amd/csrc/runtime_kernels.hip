@@ -169,13 +169,14 @@ extern "C" hipError_t va_scan_u32(uint32_t *temp, const uint32_t *in, uint32_t *
 }
 
 // ---------------------------------------------------------------------------------------------
-// Stable LSD radix sort of (key, value) pairs, 8-bit digits, 4096 pairs per workgroup:
+// Stable LSD radix sort of (key, value) pairs, 8-bit digits, 2048 pairs per workgroup (twice the
+// workgroups of 4096-pair ones: binning 0.105 vs 0.110 ms on 2.1M pairs; 1024: 0.113):
 //   histogram per (digit, workgroup) -> exclusive scan in digit-major order -> stable scatter.
 // The pair count is read on the device (min(*n_dev, cap)), so the launches are sized by the
 // capacity and need no host readback.
 
 #define RK_SORT_T 256u
-#define RK_SORT_I 16u
+#define RK_SORT_I 8u
 #define RK_SORT_B (RK_SORT_T * RK_SORT_I)
 
 __global__ __launch_bounds__(256) void va_sort_hist_kernel(const u32 *__restrict__ keys, const u32 *__restrict__ n_dev,
@@ -197,7 +198,7 @@ __global__ __launch_bounds__(256) void va_sort_hist_kernel(const u32 *__restrict
   hist[threadIdx.x * nbs + blockIdx.x] = s_h[threadIdx.x];
 }
 
-// Wave w of the workgroup ranks pairs [base + 1024 w, base + 1024 (w + 1)) in order, 64 at a time:
+// Wave w of the workgroup ranks pairs [base + 512 w, base + 512 (w + 1)) in order, 64 at a time:
 // lanes with equal digits find each other with eight ballots, and per-wave digit counters in LDS
 // carry the rank across steps. Waves are then ordered by a per-digit prefix over the waves. The
 // ranked pairs are staged in LDS in sorted order (keys, then values through the same buffer)
