@@ -477,17 +477,37 @@ extern "C" __global__ __launch_bounds__(VA_EMIT_G) void va_emit(
 // Bin bounds from the sorted keys: start[t] = first index whose key is >= t, for every tile
 // t in [0, ntiles] (so tile t's bin is [start[t], start[t + 1])), n = min(*n_dev, cap). One
 // coalesced pass over the keys: index i is the start of every tile t with
-// key[i - 1] < t <= key[i] (key[-1] = -1, key[n] = ntiles), so each entry is written once.
+// key[i - 1] < t <= key[i] (key[-1] = -1, key[n] = ntiles), so each entry is written once. A lane
+// writes a gap of up to 16 tiles itself; a longer gap (empty tiles, e.g. all tiles outside a
+// single draw's few) is written by its whole wave, 64 entries per step.
 extern "C" __global__ __launch_bounds__(256) void va_ranges(const u32 *__restrict__ keys, const u32 *__restrict__ n_dev,
                                                             u32 cap, u32 ntiles, u32 *__restrict__ start)
 {
   const u32 n = min(*n_dev, cap);
-  for(u32 i = blockIdx.x * 256u + threadIdx.x; i <= n; i += gridDim.x * 256u)
+  const u32 lane = threadIdx.x & 63u;
+  for(u32 base = blockIdx.x * 256u + (threadIdx.x & ~63u); base <= n; base += gridDim.x * 256u)
   {
-    const u32 lo = i == 0u ? 0u : keys[i - 1u] + 1u;
-    const u32 hi = i == n ? ntiles : keys[i];
-    for(u32 t = lo; t <= hi; t++)
-      start[t] = i;
+    const u32 i = base + lane;
+    u32 lo = 1u, hi = 0u;  // no tile
+    if(i <= n)
+    {
+      lo = i == 0u ? 0u : keys[i - 1u] + 1u;
+      hi = i == n ? ntiles : keys[i];
+    }
+    const bool big = hi >= lo && hi - lo >= 16u;
+    if(!big)
+      for(u32 t = lo; t <= hi; t++)
+        start[t] = i;
+    u64 bigs = __ballot(big);
+    while(bigs)
+    {
+      const u32 l = (u32)__builtin_ctzll(bigs);
+      bigs &= bigs - 1ull;
+      const u32 bl = __builtin_amdgcn_readlane(lo, l), bh = __builtin_amdgcn_readlane(hi, l);
+      const u32 bi = __builtin_amdgcn_readlane(i, l);
+      for(u32 t = bl + lane; t <= bh; t += 64u)
+        start[t] = bi;
+    }
   }
 }
 
