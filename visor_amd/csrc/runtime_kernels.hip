@@ -179,6 +179,25 @@ extern "C" hipError_t va_scan_u32(uint32_t *temp, const uint32_t *in, uint32_t *
 #define RK_SORT_I 8u
 #define RK_SORT_B (RK_SORT_T * RK_SORT_I)
 
+// Workgroup i of a sort kernel takes block rk_sort_block(i, n): workgroups are dispatched
+// round-robin over the 8 XCDs, each with its own L2, and this deals runs of 16 consecutive blocks
+// to the XCDs in turn (the tail past the last whole round keeps i). The 16 words that a run's
+// blocks write next to each other in a digit's row of the histogram (and the neighbouring runs
+// of a digit in the scatter's output) then meet in one L2 instead of as partial cache lines
+// from eight.
+__device__ __forceinline__ u32 rk_sort_block(u32 i, u32 n)
+{
+#ifdef RK_SORT_NO_XCD_RUNS
+  return i;
+#else
+  const u32 full = n & ~127u;
+  if(i >= full)
+    return i;
+  const u32 x = i & 7u, s = i >> 3;  // s-th workgroup of XCD x
+  return (((s >> 4) << 3) + x) * 16u + (s & 15u);
+#endif
+}
+
 __global__ __launch_bounds__(256) void va_sort_hist_kernel(const u32 *__restrict__ keys, const u32 *__restrict__ n_dev,
                                                            u32 cap, u32 shift, u32 *__restrict__ hist, u32 nbs)
 {
@@ -186,7 +205,8 @@ __global__ __launch_bounds__(256) void va_sort_hist_kernel(const u32 *__restrict
   s_h[threadIdx.x] = 0u;
   __syncthreads();
   const u32 n = min(*n_dev, cap);
-  const u32 base = blockIdx.x * RK_SORT_B;
+  const u32 blk = rk_sort_block(blockIdx.x, nbs);
+  const u32 base = blk * RK_SORT_B;
 #pragma unroll 4
   for(u32 i = 0; i < RK_SORT_I; i++)
   {
@@ -195,7 +215,7 @@ __global__ __launch_bounds__(256) void va_sort_hist_kernel(const u32 *__restrict
       atomicAdd(&s_h[(keys[idx] >> shift) & 255u], 1u);
   }
   __syncthreads();
-  hist[threadIdx.x * nbs + blockIdx.x] = s_h[threadIdx.x];
+  hist[threadIdx.x * nbs + blk] = s_h[threadIdx.x];
 }
 
 // Wave w of the workgroup ranks pairs [base + 512 w, base + 512 (w + 1)) in order, 64 at a time:
@@ -215,10 +235,11 @@ __global__ __launch_bounds__(256) void va_sort_scatter_kernel(const u32 *__restr
   __shared__ u32 s_w[4];
   const u32 tid = threadIdx.x, lane = tid & 63u, w = tid >> 6;
   const u32 n = min(*n_dev, cap);
-  const u32 base = blockIdx.x * RK_SORT_B;
+  const u32 blk = rk_sort_block(blockIdx.x, nbs);
+  const u32 base = blk * RK_SORT_B;
   if(base >= n)
     return;
-  s_goff[tid] = hist[tid * nbs + blockIdx.x];
+  s_goff[tid] = hist[tid * nbs + blk];
 #pragma unroll
   for(u32 k = 0; k < 4u; k++)
     s_cnt[k][tid] = 0u;
