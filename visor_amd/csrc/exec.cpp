@@ -896,6 +896,14 @@ VkResult Queue::RunPass(const std::vector<Op *> &passes, bool fused_clear, ExecS
       }
       frames.push_back(fr);
     }
+    // the device's block table: block_draw, then each block's first primitive (and the total)
+    tb.block_tab = block_draw;
+    for(uint32_t b = 0; b < (uint32_t)block_draw.size(); b++)
+    {
+      const VaDraw &v = draws[block_draw[b]];
+      tb.block_tab.push_back(v.tri_base + (b - v.block_base) * VA_SETUP_PRIMS);
+    }
+    tb.block_tab.push_back(tri_total);
     tb.version = 0;
     if(reusable)
     {
@@ -944,11 +952,10 @@ VkResult Queue::RunPass(const std::vector<Op *> &passes, bool fused_clear, ExecS
   const uint32_t nblocks = (uint32_t)block_draw.size();
   if(!Grow(&d_frames, c_frames, sizeof(VaFrame) * nframes) || !Grow(&d_draws, c_draws, sizeof(VaDraw) * std::max(ndraws, 1u)) ||
      !Grow(&d_rec, c_rec, (size_t)stride * 4 * std::max(tri_total, 1u)) || !Grow(&d_count, c_count, 4ull * (tri_total + 1)) ||
-     !Grow(&d_offs, c_offs, 4ull * (tri_total + 1)) || !Grow(&d_start, c_start, 4ull * (tile_total + 1)) ||
+     !Grow(&d_offs, c_offs, 4ull * (nblocks + 1)) || !Grow(&d_start, c_start, 4ull * (tile_total + 1)) ||
      !Grow(&d_einfo, c_einfo, 16ull * std::max(tri_total, 1u)) || !Grow(&d_stamps, c_stamps, 256) ||
-     !Grow(&d_blocks, c_blocks, 4ull * std::max<size_t>(nblocks, 1)) ||
-     !Grow(&d_part, c_part, 32ull * ((size_t)nblocks + tile_total + 1)) ||
-     !Grow(&d_temp, c_temp, 4 * va_scan_temp_words(tri_total + 1)))
+     !Grow(&d_blocks, c_blocks, 4ull * (2ull * nblocks + 1)) ||
+     !Grow(&d_part, c_part, 32ull * ((size_t)nblocks + tile_total + 1)))
     return VK_ERROR_OUT_OF_DEVICE_MEMORY;
   // d_stamps: stage stamps (u64 0-3), persistent tile counter (u32 8), finish accumulators (u64
   // 8-15) and ticket (u32 32); the accumulators and the ticket start at zero
@@ -992,10 +999,10 @@ VkResult Queue::RunPass(const std::vector<Op *> &passes, bool fused_clear, ExecS
     up_draws = draws;
     up_draws_ptr = d_draws;
   }
-  if(nblocks && (d_blocks != up_blocks_ptr || block_draw != up_blocks))
+  if(nblocks && (d_blocks != up_blocks_ptr || tb.block_tab != up_blocks))
   {
-    HIPCHK(hipMemcpyAsync(d_blocks, block_draw.data(), 4 * nblocks, hipMemcpyHostToDevice, stream));
-    up_blocks = block_draw;
+    HIPCHK(hipMemcpyAsync(d_blocks, tb.block_tab.data(), 4 * tb.block_tab.size(), hipMemcpyHostToDevice, stream));
+    up_blocks = tb.block_tab;
     up_blocks_ptr = d_blocks;
   }
   up_version = tb.version;
@@ -1003,9 +1010,16 @@ VkResult Queue::RunPass(const std::vector<Op *> &passes, bool fused_clear, ExecS
 
   // The pair buffers are sized by a capacity that only grows: the device work never waits for
   // the host. If a frame needs more pairs than the capacity, the raster launch writes nothing
-  // and the pass runs again with larger buffers.
+  // and the pass runs again with larger buffers. The launches (sort blocks, histogram words) are
+  // sized by the capacity of the pass shape (signature and tile size): its first pass runs at the
+  // queue's capacity, later ones at the pairs it needed plus an eighth, rounded up to 64K.
   if(pair_cap == 0)
     pair_cap = std::max<uint32_t>(1u << 20, 2 * tri_total);
+  if(shape_caps.size() > 4096)
+    shape_caps.clear();
+  uint32_t &shape_cap = shape_caps[sig * 8u + ts];
+  uint32_t pass_cap = shape_cap ? shape_cap : pair_cap;
+  pair_cap = std::max(pair_cap, pass_cap);
   uint32_t total_pairs = 0;
   int overflow_retries = 0;
   for(;;)
@@ -1017,15 +1031,16 @@ VkResult Queue::RunPass(const std::vector<Op *> &passes, bool fused_clear, ExecS
       return VK_ERROR_OUT_OF_DEVICE_MEMORY;
     VaDraw *dd = (VaDraw *)d_draws;
     VaFrame *df = (VaFrame *)d_frames;
-    uint32_t *rec = (uint32_t *)d_rec, *cnt = (uint32_t *)d_count, *offs = (uint32_t *)d_offs;
-    uint32_t *bd = (uint32_t *)d_blocks, *total_dev = offs + tri_total;
+    // block_pairs: (tile, primitive) pairs per setup block, then the pair total
+    uint32_t *rec = (uint32_t *)d_rec, *cnt = (uint32_t *)d_count, *block_pairs = (uint32_t *)d_offs;
+    uint32_t *bd = (uint32_t *)d_blocks, *total_dev = block_pairs + nblocks;
     // per-workgroup counter rows: setup's nblocks rows, then one raster row per tile
     uint64_t *part = (uint64_t *)d_part, *part_r = part + 4ull * nblocks;
     uint64_t *h_out = (uint64_t *)h_pinned;
-    uint32_t cap = pair_cap;
+    uint32_t cap = pass_cap;
     // The device steps of the pass in four stages, kernels only, captured together into one
-    // graph: 0 setup + count scan, 1 binning, 2 raster, 3 finish (counter sums, pair total and
-    // the stage stamps written into host memory). Setup, emit and raster stamp their own start.
+    // graph: 0 setup, 1 binning, 2 raster, 3 finish (counter sums, pair total and the stage
+    // stamps written into host memory). Setup, emit and raster stamp their own start.
     const int sort_passes = (BitsFor(tile_total) + 7) / 8;
     const bool in_alt = tri_total && (sort_passes & 1);
     uint32_t *bins = in_alt ? (uint32_t *)d_vals2 : (uint32_t *)d_vals;
@@ -1043,16 +1058,15 @@ VkResult Queue::RunPass(const std::vector<Op *> &passes, bool fused_clear, ExecS
       {
         if(tri_total)
         {
-          // the setup kernel zeroes the scan's last count and writes stamp 0 itself
+          // the setup kernel writes stamp 0 itself
           uint4 *einfo = (uint4 *)d_einfo;
-          void *args[] = {&dd, &bd, &df, (void *)&ts, &rec, &cnt, &part, &einfo, &tri_total, &stamps};
+          void *args[] = {&dd, &bd, &df, (void *)&ts, &rec, &cnt, &part, &einfo, &block_pairs, &stamps};
           VA_TRY(hipModuleLaunchKernel(mod->setup, nblocks, 1, 1, VA_SETUP_PRIMS, 1, 1, 0, stream, args, nullptr));
-          VA_TRY(va_scan_u32((uint32_t *)d_temp, cnt, offs, tri_total + 1, stream));
         }
         else
         {
           VA_TRY(va_stamp_async(stamps, stream));
-          VA_TRY(va_fill_u32_async(offs, 1, 0u, stream));
+          VA_TRY(va_fill_u32_async(total_dev, 1, 0u, stream));
         }
       }
       else if(stage == 1)
@@ -1061,8 +1075,10 @@ VkResult Queue::RunPass(const std::vector<Op *> &passes, bool fused_clear, ExecS
         {
           uint32_t *keys = (uint32_t *)d_keys, *vals = (uint32_t *)d_vals;
           uint4 *einfo = (uint4 *)d_einfo;
-          void *eargs[] = {&einfo, &offs, &tri_total, &cap, &keys, &vals, &stamps};
-          VA_TRY(hipModuleLaunchKernel(mod->emit, (tri_total + 1023) / 1024, 1, 1, 1024, 1, 1, 0, stream, eargs, nullptr));
+          uint32_t *block_first = bd + nblocks;
+          void *eargs[] = {&block_first, &block_pairs, (void *)&nblocks, &einfo, &cnt, &cap,
+                           &keys,        &vals,        &total_dev,       &stamps};
+          VA_TRY(hipModuleLaunchKernel(mod->emit, (nblocks + 3) / 4, 1, 1, 1024, 1, 1, 0, stream, eargs, nullptr));
           int alt = 0;
           VA_TRY(va_sort_pairs((uint32_t *)d_stemp, keys, vals, (uint32_t *)d_keys2, (uint32_t *)d_vals2, total_dev, cap,
                                BitsFor(tile_total), &alt, stream));
@@ -1192,14 +1208,19 @@ VkResult Queue::RunPass(const std::vector<Op *> &passes, bool fused_clear, ExecS
     HIPCHK(hipEventRecord(ev_end, stream));
     HIPCHK(hipStreamSynchronize(stream));
     total_pairs = (uint32_t)h_out[0];
-    if(total_pairs <= pair_cap)
+    if(total_pairs <= pass_cap)
       break;
     if(overflow_retries++ > 0)
     {
       LogError("bin capacity did not converge (%u pairs)", total_pairs);
       return VK_ERROR_OUT_OF_DEVICE_MEMORY;
     }
-    pair_cap = total_pairs + total_pairs / 4 + 4096;
+    pass_cap = total_pairs + total_pairs / 4 + 4096;
+    pair_cap = std::max(pair_cap, pass_cap);
+  }
+  {
+    const uint64_t want = ((uint64_t)total_pairs + total_pairs / 8 + 4096 + 0xffffu) & ~(uint64_t)0xffffu;
+    shape_cap = (uint32_t)std::min<uint64_t>(std::max<uint64_t>(want, 1u << 16), 0xffff0000u);
   }
   VaCounters c;
   memset(&c, 0, sizeof(c));

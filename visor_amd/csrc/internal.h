@@ -17,6 +17,7 @@
 #include <mutex>
 #include <string>
 #include <thread>
+#include <unordered_map>
 #include <vector>
 
 #include "../../include/visor_vk.h"
@@ -457,6 +458,7 @@ private:
   void *d_masks = nullptr;  // the raster's per-chunk pass masks (visibility pass -> shading pass)
   size_t c_masks = 0;
   uint32_t pair_cap = 0;    // (tile, primitive) pair capacity of the bin buffers
+  std::unordered_map<uint64_t, uint32_t> shape_caps;  // launch capacity per pass shape
   // last uploaded per-pass tables (skipped when a submission repeats them)
   std::vector<VaFrame> up_frames;
   std::vector<VaDraw> up_draws;
@@ -477,6 +479,7 @@ private:
     std::vector<VaDraw> draws;
     std::vector<std::shared_ptr<ShaderPair>> pairs;
     std::vector<uint32_t> block_draw;  // setup workgroup -> draw
+    std::vector<uint32_t> block_tab;   // block_draw, then each block's first primitive, then the total
     uint32_t tile_total = 0, tri_total = 0;
   };
   PassTables tables_cache;  // the last reusable pass's tables

@@ -55,6 +55,19 @@ VA_INLINE u64 va_wave_sum(u64 v)
   return v;
 }
 
+VA_INLINE u32 va_wave_exclusive_sum(u32 v, u32 lane)
+{
+  u32 incl = v;
+#pragma unroll
+  for(u32 o = 1u; o < 64u; o <<= 1)
+  {
+    const u32 t = (u32)__shfl_up((int)incl, o);
+    if(lane >= o)
+      incl += t;
+  }
+  return incl - v;
+}
+
 // Sums four counters over the workgroup and writes them as one row (every workgroup writes
 // its row, so the partials need no clearing).
 VA_INLINE void va_block_partials(u64 *row, u64 a, u64 b, u64 c, u64 e)
@@ -229,14 +242,11 @@ VA_INLINE u64 va_tile_mask(const u32 *r, i32 minx, i32 miny, i32 maxx, i32 maxy,
 extern "C" __global__ __launch_bounds__(VA_SETUP_PRIMS) void va_setup(
     const VaDraw *__restrict__ draws, const u32 *__restrict__ block_draw, const VaFrame *__restrict__ frames,
     u32 tile_shift, u32 *__restrict__ rec, u32 *__restrict__ tile_count, u64 *__restrict__ part,
-    uint4 *__restrict__ emit_info, u32 total_tris, u64 *__restrict__ stamps)
+    uint4 *__restrict__ emit_info, u32 *__restrict__ block_pairs, u64 *__restrict__ stamps)
 {
   constexpr u32 Q = VA_REC_STRIDE / 4u;
   if(blockIdx.x == 0u && threadIdx.x == 0u)
-  {
     stamps[0] = (u64)wall_clock64();
-    tile_count[total_tris] = 0u;
-  }
   const u32 di = __builtin_amdgcn_readfirstlane(block_draw[blockIdx.x]);
   const VaDraw &d = draws[di];
   const u32 prim = (blockIdx.x - d.block_base) * VA_SETUP_PRIMS + threadIdx.x;
@@ -248,6 +258,7 @@ extern "C" __global__ __launch_bounds__(VA_SETUP_PRIMS) void va_setup(
   uint4 hi[H1 > 0 ? H1 : 1];
   u32 n_in = 0, n_out = 0, n_blocks = 0;
   u64 n_tested = 0;
+  u32 cnt = 0;  // tiles of the primitive's pair set
   if(prim < d.ntris)
   {
     const VaFrame &fr = frames[d.frame];
@@ -279,7 +290,6 @@ extern "C" __global__ __launch_bounds__(VA_SETUP_PRIMS) void va_setup(
       Y[k] = va_f2i((pos[k][1] * -1.0f / pos[k][3] + 1.0f) * 0.5f * Hf);
     }
     n_in = 1;
-    u32 cnt = 0;
     // double_triarea (rasterizer.cpp:272-275), wrapping int32
     const i32 area2 = (i32)(va_wsub(X[1], X[0]) * va_wsub(Y[2], Y[0]) - va_wsub(Y[1], Y[0]) * va_wsub(X[2], X[0]));
     if(area2 != 0)
@@ -406,9 +416,22 @@ extern "C" __global__ __launch_bounds__(VA_SETUP_PRIMS) void va_setup(
       }
     }
   }
+  // the block's pair count, from which va_emit finds where its pairs start
+  __shared__ u32 s_pairs[VA_SETUP_PRIMS / 64u];
+  const u32 wave_pairs = __ockl_wfred_add_u32(cnt);
+  if((threadIdx.x & 63u) == 0u)
+    s_pairs[threadIdx.x >> 6] = wave_pairs;
   // counters: one row of partial sums per workgroup (no contended atomics), columns in
-  // VaCounters order from tris_in
+  // VaCounters order from tris_in; its barrier also completes s_pairs
   va_setup_partials(part + (u64)blockIdx.x * 4u, n_in != 0u, n_out != 0u, n_blocks, n_tested);
+  if(threadIdx.x == 0u)
+  {
+    u32 t = 0;
+#pragma unroll
+    for(u32 k = 0; k < VA_SETUP_PRIMS / 64u; k++)
+      t += s_pairs[k];
+    block_pairs[blockIdx.x] = t;
+  }
 }
 
 // Position of the k-th set bit of m (k < popcount(m)).
@@ -428,49 +451,97 @@ VA_INLINE u32 va_nth_bit(u64 m, u32 k)
   return pos;
 }
 
-// Pair generation, load-balanced: workgroup b expands primitives [G b, G b + G), G = 1024 (one
-// thread per primitive; the work per primitive is small, and 1024-wide groups need a quarter of
-// the workgroups of 256-wide ones: 31 vs 46 us on 1.85M triangles). offsets is the exclusive scan of the per-primitive tile counts
-// (total_tris + 1 entries, the last one the pair total); the group's offsets and emit descriptors
-// are staged in LDS, and each thread takes pairs of the group by stride, finding its primitive by
-// a binary search over the G + 1 offsets. Pairs at or beyond cap are not written (the host sees
+// Pair generation, load-balanced: workgroup w expands the primitives of setup blocks [4 w,
+// 4 w + 4), a contiguous range of at most 1024 primitives, one thread per primitive (the work per
+// primitive is small, and 1024-wide groups need a quarter of the workgroups of 256-wide ones: 31
+// vs 46 us on 1.85M triangles). The group's first pair slot is the sum of the pair counts of the
+// blocks before it (setup's block_pairs; workgroup 0 also sums them all into *pair_total); the
+// primitives' offsets inside the group are an exclusive scan in LDS, next to their emit
+// descriptors, and each thread takes pairs of the group by stride, finding its primitive by a
+// binary search over the G + 1 offsets. Pairs at or beyond cap are not written (the host sees
 // the total and grows the buffers). Workgroup 0 stamps the start of binning.
 #define VA_EMIT_G 1024u
 extern "C" __global__ __launch_bounds__(VA_EMIT_G) void va_emit(
-    const uint4 *__restrict__ emit_info, const u32 *__restrict__ offsets, u32 total_tris, u32 cap,
-    u32 *__restrict__ keys, u32 *__restrict__ vals, u64 *__restrict__ stamps)
+    const u32 *__restrict__ block_first, const u32 *__restrict__ block_pairs, u32 nblocks,
+    const uint4 *__restrict__ emit_info, const u32 *__restrict__ tile_count, u32 cap, u32 *__restrict__ keys,
+    u32 *__restrict__ vals, u32 *__restrict__ pair_total, u64 *__restrict__ stamps)
 {
+  static_assert(VA_EMIT_G == 4u * VA_SETUP_PRIMS, "four setup blocks per group");
   __shared__ u32 s_off[VA_EMIT_G + 1u];
   __shared__ uint4 s_info[VA_EMIT_G];
-  const u32 t0 = blockIdx.x * VA_EMIT_G;
-  if(blockIdx.x == 0u && threadIdx.x == 0u)
+  __shared__ u32 s_w[2][VA_EMIT_G / 64u];
+  const u32 tid = threadIdx.x, lane = tid & 63u, w = tid >> 6;
+  if(blockIdx.x == 0u && tid == 0u)
     stamps[1] = (u64)wall_clock64();
-  s_off[threadIdx.x] = offsets[min(t0 + threadIdx.x, total_tris)];
-  if(threadIdx.x == 0u)
-    s_off[VA_EMIT_G] = offsets[min(t0 + VA_EMIT_G, total_tris)];
-  __syncthreads();
-  // descriptors only of primitives that emit (the others were never written)
-  if(t0 + threadIdx.x < total_tris && s_off[threadIdx.x + 1u] != s_off[threadIdx.x])
-    s_info[threadIdx.x] = emit_info[t0 + threadIdx.x];
-  __syncthreads();
-  const u32 pend = min(s_off[VA_EMIT_G], cap);
-  for(u32 p = s_off[0] + threadIdx.x; p < pend; p += VA_EMIT_G)
+  // block_first[b] = first primitive of setup block b, block_first[nblocks] = the total
+  const u32 b0 = blockIdx.x * 4u;
+  const u32 p0 = block_first[b0], p1 = block_first[min(b0 + 4u, nblocks)];
+  // pairs of the blocks before this group (workgroup 0: of all blocks, the total)
+  const u32 lim = blockIdx.x == 0u ? nblocks : b0;
+  u32 pre = 0;
+  for(u32 bb = 0; bb < lim; bb += 8u * VA_EMIT_G)
   {
-    // last i with s_off[i] <= p (the steps sum to G - 1, so i + step stays below G)
+    // eight independent loads in flight per thread
+    u32 x[8];
+#pragma unroll
+    for(u32 k = 0; k < 8u; k++)
+    {
+      const u32 b = bb + k * VA_EMIT_G + tid;
+      x[k] = b < lim ? block_pairs[b] : 0u;
+    }
+#pragma unroll
+    for(u32 k = 0; k < 8u; k++)
+      pre += x[k];
+  }
+  const u32 g = p0 + tid;
+  const u32 c = g < p1 ? tile_count[g] : 0u;
+  if(c != 0u)
+    s_info[tid] = emit_info[g];
+  const u32 ex = va_wave_exclusive_sum(c, lane);
+  const u32 wpre = __ockl_wfred_add_u32(pre);
+  if(lane == 63u)
+  {
+    s_w[0][w] = ex + c;
+    s_w[1][w] = wpre;
+  }
+  __syncthreads();
+  u32 off = 0, L = 0, base = 0;
+#pragma unroll
+  for(u32 k = 0; k < VA_EMIT_G / 64u; k++)
+  {
+    const u32 x = s_w[0][k];
+    off += k < w ? x : 0u;
+    L += x;
+    base += s_w[1][k];
+  }
+  if(blockIdx.x == 0u)
+  {
+    if(tid == 0u)
+      *pair_total = base;
+    base = 0u;
+  }
+  s_off[tid] = off + ex;
+  if(tid == 0u)
+    s_off[VA_EMIT_G] = L;
+  __syncthreads();
+  const u32 qend = base < cap ? min(L, cap - base) : 0u;
+  for(u32 q = tid; q < qend; q += VA_EMIT_G)
+  {
+    // last i with s_off[i] <= q (the steps sum to G - 1, so i + step stays below G)
     u32 i = 0;
 #pragma unroll
     for(u32 step = VA_EMIT_G / 2u; step > 0u; step >>= 1)
-      if(s_off[i + step] <= p)
+      if(s_off[i + step] <= q)
         i += step;
     const uint4 e = s_info[i];
     const u64 mask = (u64)e.z | ((u64)e.w << 32);
     // the k-th pair of the primitive is the k-th tile of its mask (the k-th tile of the
     // rectangle when the mask is all ones)
-    const u32 k = mask == ~0ull ? p - s_off[i] : va_nth_bit(mask, p - s_off[i]);
+    const u32 k = mask == ~0ull ? q - s_off[i] : va_nth_bit(mask, q - s_off[i]);
     const u32 nx = e.y & 0xffffu, tiles_x = e.y >> 16;
     const u32 row = k / nx;
-    keys[p] = e.x + row * tiles_x + (k - row * nx);
-    vals[p] = t0 + i;
+    keys[base + q] = e.x + row * tiles_x + (k - row * nx);
+    vals[base + q] = p0 + i;
   }
 }
 
@@ -740,18 +811,6 @@ VA_INLINE u32 va_transpose32x64(u64 x, u32 lane)
   return v;
 }
 
-VA_INLINE u32 va_wave_exclusive_sum(u32 v, u32 lane)
-{
-  u32 incl = v;
-#pragma unroll
-  for(u32 o = 1u; o < 64u; o <<= 1)
-  {
-    const u32 t = (u32)__shfl_up((int)incl, o);
-    if(lane >= o)
-      incl += t;
-  }
-  return incl - v;
-}
 
 // Coverage only (the integer part of va_bary).
 VA_INLINE bool va_covered(const u32 *__restrict__ r, i32 x, i32 y)
