@@ -39,7 +39,7 @@ int va_sort_pairs(uint32_t *temp, uint32_t *keys, uint32_t *vals, uint32_t *keys
 
 /* No counterpart in the reference. For callers that cannot allocate device memory (code under
  * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc): va_sort_pairs without
- * a workspace argument. The histogram and scan words live in a device array of the library sized
+ * a workspace argument. The histogram words and digit totals live in a device array of the library sized
  * for VA_SORT_NOALLOC_MAX_PAIRS pairs (8 MiB per GPU, held until the process ends); the call
  * allocates nothing. cap above the maximum returns a nonzero error before any device work. Every
  * call on one GPU shares that GPU's array, so calls that overlap on one GPU (two streams) race. */

@@ -214,7 +214,14 @@ static std::string ModuleSource(const std::vector<std::shared_ptr<ShaderPair>> &
   }
   rec_stride = (VA_REC_HDR + 3 * maxnv + 3) & ~3u;
   std::ostringstream s;
-  s << kSrcDeviceAbi << "\n" << kSrcPrelude << "\n";
+  // VISOR_PRELUDE_SOURCE / VISOR_KERNEL_SOURCE: development overrides of the built-in prelude and
+  // kernel files (same interface)
+  std::vector<char> override_prelude;
+  const char *ps = getenv("VISOR_PRELUDE_SOURCE");
+  if(ps && ReadFile(ps, override_prelude))
+    s << kSrcDeviceAbi << "\n" << std::string(override_prelude.begin(), override_prelude.end()) << "\n";
+  else
+    s << kSrcDeviceAbi << "\n" << kSrcPrelude << "\n";
   s << "#define VA_NSLOTS " << pairs.size() << "\n";
   s << "#define VA_MAXNV " << maxnv << "\n";
   s << "#define VA_REC_STRIDE " << rec_stride << "u\n";
@@ -258,7 +265,6 @@ static std::string ModuleSource(const std::vector<std::shared_ptr<ShaderPair>> &
   for(size_t i = 0; i < pairs.size(); i++)
     s << "    case " << i << "u: return " << pairs[i]->fs.fn_name << "(d, vary, b, out);\n";
   s << "    default: return false;\n  }\n}\n";
-  // VISOR_KERNEL_SOURCE: development override of the built-in kernel file (same interface)
   std::vector<char> override_src;
   const char *ks = getenv("VISOR_KERNEL_SOURCE");
   if(ks && ReadFile(ks, override_src))

@@ -17,7 +17,7 @@ hipError_t va_scan_u32(uint32_t *temp, const uint32_t *in, uint32_t *out, uint32
 size_t va_sort_temp_words(uint32_t cap);
 hipError_t va_sort_pairs(uint32_t *temp, uint32_t *keys, uint32_t *vals, uint32_t *keys_alt, uint32_t *vals_alt,
                          const uint32_t *n_dev, uint32_t cap, int bits, int *result_in_alt, hipStream_t stream);
-/* va_sort_pairs without a workspace: the histogram and scan words come from a device array of
+/* va_sort_pairs without a workspace: the histogram words and digit totals come from a device array of
  * the library sized for VA_SORT_NOALLOC_MAX_PAIRS pairs (8 MiB), so the call allocates nothing.
  * cap above the maximum returns hipErrorInvalidValue. Calls on one device share the array:
  * calls that overlap on one device race. */

@@ -171,7 +171,8 @@ extern "C" hipError_t va_scan_u32(uint32_t *temp, const uint32_t *in, uint32_t *
 // ---------------------------------------------------------------------------------------------
 // Stable LSD radix sort of (key, value) pairs, 8-bit digits, 2048 pairs per workgroup (twice the
 // workgroups of 4096-pair ones: binning 0.105 vs 0.110 ms on 2.1M pairs; 1024: 0.113):
-//   histogram per (digit, workgroup) -> exclusive scan in digit-major order -> stable scatter.
+//   histogram per (digit, workgroup) -> exclusive scan of each digit's row (one workgroup per
+//   digit; the scatter adds the scan of the digit totals) -> stable scatter.
 // The pair count is read on the device (min(*n_dev, cap)), so the launches are sized by the
 // capacity and need no host readback.
 
@@ -226,7 +227,8 @@ __global__ __launch_bounds__(256) void va_sort_hist_kernel(const u32 *__restrict
 __global__ __launch_bounds__(256) void va_sort_scatter_kernel(const u32 *__restrict__ kin, const u32 *__restrict__ vin,
                                                               u32 *__restrict__ kout, u32 *__restrict__ vout,
                                                               const u32 *__restrict__ n_dev, u32 cap, u32 shift,
-                                                              const u32 *__restrict__ hist, u32 nbs)
+                                                              const u32 *__restrict__ hist, const u32 *__restrict__ tot,
+                                                              u32 nbs)
 {
   __shared__ u32 s_cnt[4][256];
   __shared__ u32 s_goff[256];
@@ -239,7 +241,12 @@ __global__ __launch_bounds__(256) void va_sort_scatter_kernel(const u32 *__restr
   const u32 base = blk * RK_SORT_B;
   if(base >= n)
     return;
-  s_goff[tid] = hist[tid * nbs + blk];
+  // this block's first slot for digit tid: the pairs of lower digits (an exclusive scan of the
+  // digit totals) + those of digit tid in earlier blocks (the row scan)
+  {
+    u32 all;
+    s_goff[tid] = rk_block_exclusive<4>(tot[tid], s_w, all) + hist[tid * nbs + blk];
+  }
 #pragma unroll
   for(u32 k = 0; k < 4u; k++)
     s_cnt[k][tid] = 0u;
@@ -323,13 +330,43 @@ __global__ __launch_bounds__(256) void va_sort_scatter_kernel(const u32 *__restr
   }
 }
 
+// Exclusive scan of each digit's row of the histogram in place (row d = the nbs blocks' counts of
+// digit d, contiguous in the digit-major layout), and tot[d] = the row's total; the scatter adds
+// the scan of the 256 totals itself. One workgroup per digit, eight counts per thread per step.
+#define RK_ROW_I 8u
+__global__ __launch_bounds__(256) void va_sort_rowscan_kernel(u32 *__restrict__ hist, u32 nbs, u32 *__restrict__ tot)
+{
+  __shared__ u32 s_w[4];
+  u32 *row = hist + (u64)blockIdx.x * nbs;
+  u32 carry = 0;
+  for(u32 c = 0; c < nbs; c += 256u * RK_ROW_I)
+  {
+    const u32 i0 = c + threadIdx.x * RK_ROW_I;
+    u32 v[RK_ROW_I], t = 0;
+#pragma unroll
+    for(u32 k = 0; k < RK_ROW_I; k++)
+    {
+      v[k] = i0 + k < nbs ? row[i0 + k] : 0u;
+      t += v[k];
+    }
+    u32 total;
+    u32 run = carry + rk_block_exclusive<4>(t, s_w, total);
+#pragma unroll
+    for(u32 k = 0; k < RK_ROW_I; k++)
+    {
+      if(i0 + k < nbs)
+        row[i0 + k] = run;
+      run += v[k];
+    }
+    carry += total;
+  }
+  if(threadIdx.x == 0u)
+    tot[blockIdx.x] = carry;
+}
+
 static u32 sort_blocks(u32 cap) { return (cap + RK_SORT_B - 1) / RK_SORT_B; }
 
-extern "C" size_t va_sort_temp_words(uint32_t cap)
-{
-  const u32 nh = 256u * sort_blocks(cap);
-  return (size_t)nh + va_scan_temp_words(nh);
-}
+extern "C" size_t va_sort_temp_words(uint32_t cap) { return 256u * (size_t)sort_blocks(cap) + 256u; }
 
 extern "C" hipError_t va_sort_pairs(uint32_t *temp, uint32_t *keys, uint32_t *vals, uint32_t *keys_alt,
                                     uint32_t *vals_alt, const uint32_t *n_dev, uint32_t cap, int bits,
@@ -339,19 +376,16 @@ extern "C" hipError_t va_sort_pairs(uint32_t *temp, uint32_t *keys, uint32_t *va
   if(cap == 0)
     return hipSuccess;
   const u32 nbs = sort_blocks(cap), nh = 256u * nbs;
-  u32 *hist = temp, *scan_tmp = temp + nh;
+  u32 *hist = temp, *tot = temp + nh;
   u32 *ki = keys, *vi = vals, *ko = keys_alt, *vo = vals_alt;
   for(int shift = 0; shift < bits; shift += 8)
   {
     (void)hipGetLastError();  // report this launch's error only
     hipLaunchKernelGGL(va_sort_hist_kernel, dim3(nbs), dim3(RK_SORT_T), 0, stream, (const u32 *)ki, n_dev, cap,
                        (u32)shift, hist, nbs);
-    hipError_t e = va_scan_u32(scan_tmp, hist, hist, nh, stream);
-    if(e != hipSuccess)
-      return e;
-    (void)hipGetLastError();  // report this launch's error only
+    hipLaunchKernelGGL(va_sort_rowscan_kernel, dim3(256), dim3(256), 0, stream, hist, nbs, tot);
     hipLaunchKernelGGL(va_sort_scatter_kernel, dim3(nbs), dim3(RK_SORT_T), 0, stream, (const u32 *)ki,
-                       (const u32 *)vi, ko, vo, n_dev, cap, (u32)shift, (const u32 *)hist, nbs);
+                       (const u32 *)vi, ko, vo, n_dev, cap, (u32)shift, (const u32 *)hist, (const u32 *)tot, nbs);
     u32 *t = ki; ki = ko; ko = t;
     t = vi; vi = vo; vo = t;
     *result_in_alt ^= 1;
@@ -360,17 +394,15 @@ extern "C" hipError_t va_sort_pairs(uint32_t *temp, uint32_t *keys, uint32_t *va
 }
 
 // ---------------------------------------------------------------------------------------------
-// The same sort without a workspace argument and without allocating: the histogram and scan
-// words live in `scratch`, a file-scope device array sized for VA_SORT_NOALLOC_MAX_PAIRS pairs.
+// The same sort without a workspace argument and without allocating: the histogram words and
+// digit totals live in `scratch`, a file-scope device array sized for VA_SORT_NOALLOC_MAX_PAIRS pairs.
 // Host code reaches the array through the address hipGetSymbolAddress returns for the current
 // device (the array's name is only a host-side stand-in), looked up once per device on the first
 // call there. Every call on one device shares that device's array; nothing in it needs clearing
 // (the histogram kernel writes every word the scan and scatter read), and calls that overlap on
 // one device (two streams) race on it.
 
-static constexpr size_t kNoallocSortWords =
-    256u * ((VA_SORT_NOALLOC_MAX_PAIRS + RK_SORT_B - 1) / RK_SORT_B) +
-    (256u * ((VA_SORT_NOALLOC_MAX_PAIRS + RK_SORT_B - 1) / RK_SORT_B) + RK_SCAN_B - 1) / RK_SCAN_B + 64u;
+static constexpr size_t kNoallocSortWords = 256u * ((VA_SORT_NOALLOC_MAX_PAIRS + RK_SORT_B - 1) / RK_SORT_B) + 256u;
 static __device__ __align__(256) uint32_t scratch[kNoallocSortWords];
 
 #define RK_MAX_DEVICES 64
