@@ -602,12 +602,10 @@ VA_INLINE float va_half_val(u32 b) { return (float)__builtin_bit_cast(_Float16, 
 
 // Perspective-correct barycentrics at integer corner (x, y) from the record
 // (rasterizer.cpp:282-310, 551-588). Returns coverage; writes the pre-perspective depth.
-VA_INLINE bool va_bary(const u32 *__restrict__ r, i32 x, i32 y, float b[4], float *depth)
+// The barycentric weights from a pixel's two integer edge functions ux, uy (b1 and b2; b0 =
+// area - ux - uy), as va_bary computes them.
+VA_INLINE bool va_bary_uv(const u32 *__restrict__ r, u32 ux, u32 uy, float b[4], float *depth)
 {
-  const i32 PAx = (i32)(r[VA_R_X0] - (u32)x);
-  const i32 PAy = (i32)(r[VA_R_Y0] - (u32)y);
-  const u32 ux = r[VA_R_ACX] * (u32)PAy - r[VA_R_ACY] * (u32)PAx;
-  const u32 uy = (u32)PAx * r[VA_R_ABY] - (u32)PAy * r[VA_R_ABX];
   const i32 b0 = (i32)(r[VA_R_AREA] - (ux + uy)), b1 = (i32)ux, b2 = (i32)uy;
   const bool covered = b0 >= 0 && b1 >= 0 && b2 >= 0;
   const float invarea = va_u2f(r[VA_R_INVAREA]);
@@ -626,6 +624,33 @@ VA_INLINE bool va_bary(const u32 *__restrict__ r, i32 x, i32 y, float b[4], floa
   b[2] = n2 * invlen;
   b[3] = 0.0f;
   return covered;
+}
+
+VA_INLINE void va_edges(const u32 *__restrict__ r, i32 x, i32 y, u32 &ux, u32 &uy)
+{
+  const u32 PAx = r[VA_R_X0] - (u32)x, PAy = r[VA_R_Y0] - (u32)y;
+  ux = r[VA_R_ACX] * PAy - r[VA_R_ACY] * PAx;
+  uy = PAx * r[VA_R_ABY] - PAy * r[VA_R_ABX];
+}
+
+VA_INLINE bool va_bary(const u32 *__restrict__ r, i32 x, i32 y, float b[4], float *depth)
+{
+  u32 ux, uy;
+  va_edges(r, x, y, ux, uy);
+  return va_bary_uv(r, ux, uy, b, depth);
+}
+
+// The weights at the pixel and at its right and lower neighbours. The neighbours' edge functions
+// are the pixel's stepped by one (x + 1 adds ACY to ux and -ABY to uy, y + 1 adds -ACX and ABX):
+// the same values modulo 2^32 as evaluating them there, without the four 32-bit products.
+VA_INLINE bool va_bary3(const u32 *__restrict__ r, i32 x, i32 y, float b[3][4], float *depth)
+{
+  u32 ux, uy;
+  va_edges(r, x, y, ux, uy);
+  const bool cov = va_bary_uv(r, ux, uy, b[0], depth);
+  va_bary_uv(r, ux + r[VA_R_ACY], uy - r[VA_R_ABY], b[1], nullptr);
+  va_bary_uv(r, ux - r[VA_R_ACX], uy + r[VA_R_ABX], b[2], nullptr);
+  return cov;
 }
 
 VA_INLINE float va_factor_value(u32 code, float a)
@@ -943,14 +968,17 @@ VA_INLINE void va_raster_tile(const VaDraw *__restrict__ draws, const VaFrame *_
       if(maxx <= bx || minx >= bx + 8 || maxy <= by || miny >= by + 8) continue;
       float b[3][4];
       float pixdepth;
-      bool cov = va_bary(r, px, py, b[0], &pixdepth);
+      u32 ux, uy;
+      va_edges(r, px, py, ux, uy);
+      bool cov = va_bary_uv(r, ux, uy, b[0], &pixdepth);
       cov = cov && inside && px >= minx && px < maxx && py >= miny && py < maxy;
       if(__ballot(cov) == 0ull || !cov) continue;
       const VaDraw &d = draws[r[VA_R_DRAW] & VA_DRAW_MASK];
       n_cov++;
       if(d.depth_op != 7u && has_depth && !va_depth_pass(d.depth_op, pixdepth, dep)) continue;
-      va_bary(r, px + 1, py, b[1], nullptr);
-      va_bary(r, px, py + 1, b[2], nullptr);
+      // the right and lower neighbours: the edge functions stepped by one
+      va_bary_uv(r, ux + r[VA_R_ACY], uy - r[VA_R_ABY], b[1], nullptr);
+      va_bary_uv(r, ux - r[VA_R_ACX], uy + r[VA_R_ABX], b[2], nullptr);
       float o[4];
       if(!va_fs(d.slot, d, (const float *)(r + VA_REC_HDR), b, o)) continue;
       n_pass++;
@@ -1164,10 +1192,10 @@ VA_INLINE void va_raster_tile(const VaDraw *__restrict__ draws, const VaFrame *_
           const u32 *r = s_rec + k * LS;
           const i32 x = bx + (i32)(p & 7u), y = by + (i32)(p >> 3);
           float b[3][4];
-          va_bary(r, x, y, b[0], nullptr);
 #if VA_FS_DERIV
-          va_bary(r, x + 1, y, b[1], nullptr);
-          va_bary(r, x, y + 1, b[2], nullptr);
+          va_bary3(r, x, y, b, nullptr);
+#else
+          va_bary(r, x, y, b[0], nullptr);
 #endif
           const VaDraw &d = draws[r[VA_R_DRAW] & VA_DRAW_MASK];
           float o[4];
