@@ -92,8 +92,8 @@ Queue::~Queue()
     qcv.notify_all();
     worker.join();
   }
-  void *bufs[] = {d_frames, d_draws, d_rec, d_count, d_offs, d_keys, d_vals, d_keys2, d_vals2, d_start, d_end, d_temp,
-                  d_blocks, d_part, d_stemp, d_einfo, d_stamps, d_masks};
+  void *bufs[] = {d_frames, d_draws, d_rec,  d_count, d_bpairs, d_keys,  d_vals,   d_keys2,
+                  d_vals2,  d_start, d_blocks, d_part, d_stemp, d_einfo, d_stamps, d_masks};
   for(void *b : bufs)
     if(b)
       hipFree(b);
@@ -952,7 +952,7 @@ VkResult Queue::RunPass(const std::vector<Op *> &passes, bool fused_clear, ExecS
   const uint32_t nblocks = (uint32_t)block_draw.size();
   if(!Grow(&d_frames, c_frames, sizeof(VaFrame) * nframes) || !Grow(&d_draws, c_draws, sizeof(VaDraw) * std::max(ndraws, 1u)) ||
      !Grow(&d_rec, c_rec, (size_t)stride * 4 * std::max(tri_total, 1u)) || !Grow(&d_count, c_count, 4ull * (tri_total + 1)) ||
-     !Grow(&d_offs, c_offs, 4ull * (nblocks + 1)) || !Grow(&d_start, c_start, 4ull * (tile_total + 1)) ||
+     !Grow(&d_bpairs, c_bpairs, 4ull * (nblocks + 1)) || !Grow(&d_start, c_start, 4ull * (tile_total + 1)) ||
      !Grow(&d_einfo, c_einfo, 16ull * std::max(tri_total, 1u)) || !Grow(&d_stamps, c_stamps, 256) ||
      !Grow(&d_blocks, c_blocks, 4ull * (2ull * nblocks + 1)) ||
      !Grow(&d_part, c_part, 32ull * ((size_t)nblocks + tile_total + 1)))
@@ -1032,7 +1032,7 @@ VkResult Queue::RunPass(const std::vector<Op *> &passes, bool fused_clear, ExecS
     VaDraw *dd = (VaDraw *)d_draws;
     VaFrame *df = (VaFrame *)d_frames;
     // block_pairs: (tile, primitive) pairs per setup block, then the pair total
-    uint32_t *rec = (uint32_t *)d_rec, *cnt = (uint32_t *)d_count, *block_pairs = (uint32_t *)d_offs;
+    uint32_t *rec = (uint32_t *)d_rec, *cnt = (uint32_t *)d_count, *block_pairs = (uint32_t *)d_bpairs;
     uint32_t *bd = (uint32_t *)d_blocks, *total_dev = block_pairs + nblocks;
     // per-workgroup counter rows: setup's nblocks rows, then one raster row per tile
     uint64_t *part = (uint64_t *)d_part, *part_r = part + 4ull * nblocks;
@@ -1141,8 +1141,8 @@ VkResult Queue::RunPass(const std::vector<Op *> &passes, bool fused_clear, ExecS
       key.dims[5] = cap;
       key.dims[6] = fused_clear ? 1u : 0u;
       key.dims[7] = cfg.persistent;
-      void *bufs[] = {d_frames, d_draws, d_rec,   d_count, d_offs,   d_keys,   d_vals,  d_keys2, d_vals2,
-                      d_stemp,  d_temp,  d_start, d_einfo, d_blocks, d_part,   d_stamps, d_masks, h_pinned};
+      void *bufs[] = {d_frames, d_draws, d_rec,   d_count, d_bpairs,   d_keys,   d_vals,  d_keys2, d_vals2,
+                      d_stemp,  d_start, d_einfo, d_blocks, d_part, d_stamps, d_masks, h_pinned};
       static_assert(sizeof(bufs) <= sizeof(key.bufs), "graph key holds every buffer");
       memcpy(key.bufs, bufs, sizeof(bufs));
       for(size_t i = 0; i < graph_cache.size(); i++)

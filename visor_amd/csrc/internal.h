@@ -443,9 +443,11 @@ private:
   hipEvent_t ev_begin = nullptr, ev_end = nullptr;
   bool drained = false;
   // growable device scratch
-  void *d_frames = nullptr, *d_draws = nullptr, *d_rec = nullptr, *d_count = nullptr, *d_offs = nullptr;
+  // d_count: tiles per primitive; d_bpairs: pairs per setup block, then the pair total; d_start:
+  // bin starts; d_blocks: setup block -> draw, then each block's first primitive
+  void *d_frames = nullptr, *d_draws = nullptr, *d_rec = nullptr, *d_count = nullptr, *d_bpairs = nullptr;
   void *d_keys = nullptr, *d_vals = nullptr, *d_keys2 = nullptr, *d_vals2 = nullptr;
-  void *d_start = nullptr, *d_end = nullptr, *d_temp = nullptr, *d_blocks = nullptr;
+  void *d_start = nullptr, *d_blocks = nullptr;
   size_t c_blocks = 0;
   void *d_part = nullptr;  // per-workgroup counter partials
   size_t c_part = 0;
@@ -501,8 +503,8 @@ private:
     hipGraphExec_t exec[1] = {nullptr};  // setup, scan, binning, raster and finish
   };
   std::vector<GraphSet> graph_cache;  // most recent first, at most 8
-  size_t c_frames = 0, c_draws = 0, c_rec = 0, c_count = 0, c_offs = 0, c_keys = 0, c_vals = 0, c_keys2 = 0,
-         c_vals2 = 0, c_start = 0, c_end = 0, c_temp = 0;
+  size_t c_frames = 0, c_draws = 0, c_rec = 0, c_count = 0, c_bpairs = 0, c_keys = 0, c_vals = 0, c_keys2 = 0,
+         c_vals2 = 0, c_start = 0;
   uint32_t *h_pinned = nullptr;  // small pinned staging for readbacks
 
   // async worker
