@@ -809,7 +809,7 @@ void Queue::RecordTuning(uint64_t sig, const TileConfig &cfg, double ms)
   }
 }
 
-VkResult Queue::RunPass(const std::vector<Op *> &passes, bool fused_clear, ExecStats &st)
+VkResult Queue::RunPass(const std::vector<Op *> &passes, bool fused_clear, ExecStats &st, PassLaunch *deferred)
 {
   // A pass that stops part way may leave the finish kernel's accumulators and ticket mid-count:
   // every early return makes the next pass zero them again.
@@ -919,6 +919,15 @@ VkResult Queue::RunPass(const std::vector<Op *> &passes, bool fused_clear, ExecS
     any_clear |= f.clear_flags != 0;
   if(tri_total == 0 && !any_clear)
     return VK_SUCCESS;
+  // Only a pass with primitives (whose emit kernel sees the overflow flag) and a chosen launch
+  // shape is launched without waiting; another one waits for the passes still running first.
+  const bool can_defer = !tuning && tri_total > 0;
+  if(deferred && !can_defer && !inflight.empty())
+  {
+    deferred->drain_first = true;
+    reset_guard.done = true;
+    return VK_SUCCESS;
+  }
   std::shared_ptr<PipelineModule> mod;
   if(!pairs.empty())
   {
@@ -958,11 +967,18 @@ VkResult Queue::RunPass(const std::vector<Op *> &passes, bool fused_clear, ExecS
      !Grow(&d_part, c_part, 32ull * ((size_t)nblocks + tile_total + 1)))
     return VK_ERROR_OUT_OF_DEVICE_MEMORY;
   // d_stamps: stage stamps (u64 0-3), persistent tile counter (u32 8), finish accumulators (u64
-  // 8-15) and ticket (u32 32); the accumulators and the ticket start at zero
+  // 8-15), ticket (u32 32) and overflow flag (u32 34); the accumulators, the ticket and the flag
+  // start at zero. The flag is zeroed only for a new allocation (afterwards the host clears it
+  // when it has seen the overflow): a pass launched behind one still running must see its flag.
   if(d_stamps != stamps_zeroed)
   {
-    HIPCHK(va_fill_u32_async(d_stamps, 64, 0u, stream));
+    HIPCHK(va_fill_u32_async(d_stamps, 34, 0u, stream));
     stamps_zeroed = d_stamps;
+  }
+  if(d_stamps != poison_zeroed)
+  {
+    HIPCHK(va_fill_u32_async((uint32_t *)d_stamps + 34, 1, 0u, stream));
+    poison_zeroed = d_stamps;
   }
   if(!h_pinned)  // written by the finish kernel, read by the host after the stream completes
     HIPCHK(hipHostMalloc((void **)&h_pinned, 4096, hipHostMallocCoherent));
@@ -1022,6 +1038,10 @@ VkResult Queue::RunPass(const std::vector<Op *> &passes, bool fused_clear, ExecS
   pair_cap = std::max(pair_cap, pass_cap);
   uint32_t total_pairs = 0;
   int overflow_retries = 0;
+  // output words of this launch in pinned host memory: two slots, so that a pass can be
+  // launched while the previous one's results still wait to be read
+  const int slot = inflight.empty() ? 0 : 1 - inflight.back().slot;
+  const bool defer = deferred && can_defer;
   for(;;)
   {
     if(!Grow(&d_keys, c_keys, 4ull * pair_cap) || !Grow(&d_vals, c_vals, 4ull * pair_cap) ||
@@ -1036,7 +1056,8 @@ VkResult Queue::RunPass(const std::vector<Op *> &passes, bool fused_clear, ExecS
     uint32_t *bd = (uint32_t *)d_blocks, *total_dev = block_pairs + nblocks;
     // per-workgroup counter rows: setup's nblocks rows, then one raster row per tile
     uint64_t *part = (uint64_t *)d_part, *part_r = part + 4ull * nblocks;
-    uint64_t *h_out = (uint64_t *)h_pinned;
+    uint64_t *h_out = (uint64_t *)h_pinned + 16 * slot;
+    uint32_t *poison = (uint32_t *)d_stamps + 34;
     uint32_t cap = pass_cap;
     // The device steps of the pass in four stages, kernels only, captured together into one
     // graph: 0 setup, 1 binning, 2 raster, 3 finish (counter sums, pair total and the stage
@@ -1076,8 +1097,8 @@ VkResult Queue::RunPass(const std::vector<Op *> &passes, bool fused_clear, ExecS
           uint32_t *keys = (uint32_t *)d_keys, *vals = (uint32_t *)d_vals;
           uint4 *einfo = (uint4 *)d_einfo;
           uint32_t *block_first = bd + nblocks;
-          void *eargs[] = {&block_first, &block_pairs, (void *)&nblocks, &einfo, &cnt, &cap,
-                           &keys,        &vals,        &total_dev,       &stamps};
+          void *eargs[] = {&block_first, &block_pairs, (void *)&nblocks, &einfo,  &cnt, &cap,
+                           &keys,        &vals,        &total_dev,       &stamps, &poison};
           VA_TRY(hipModuleLaunchKernel(mod->emit, (nblocks + 3) / 4, 1, 1, 1024, 1, 1, 0, stream, eargs, nullptr));
           int alt = 0;
           VA_TRY(va_sort_pairs((uint32_t *)d_stemp, keys, vals, (uint32_t *)d_keys2, (uint32_t *)d_vals2, total_dev, cap,
@@ -1142,7 +1163,7 @@ VkResult Queue::RunPass(const std::vector<Op *> &passes, bool fused_clear, ExecS
       key.dims[6] = fused_clear ? 1u : 0u;
       key.dims[7] = cfg.persistent;
       void *bufs[] = {d_frames, d_draws, d_rec,   d_count, d_bpairs,   d_keys,   d_vals,  d_keys2, d_vals2,
-                      d_stemp,  d_start, d_einfo, d_blocks, d_part, d_stamps, d_masks, h_pinned};
+                      d_stemp,  d_start, d_einfo, d_blocks, d_part, d_stamps, d_masks, h_out};
       static_assert(sizeof(bufs) <= sizeof(key.bufs), "graph key holds every buffer");
       memcpy(key.bufs, bufs, sizeof(bufs));
       for(size_t i = 0; i < graph_cache.size(); i++)
@@ -1194,6 +1215,14 @@ VkResult Queue::RunPass(const std::vector<Op *> &passes, bool fused_clear, ExecS
         }
       }
     }
+    const bool deferring = defer;
+    if(deferring)
+    {
+      for(int k = 0; k < 2; k++)
+        if(!ev_slot[slot][k])
+          HIPCHK(hipEventCreate(&ev_slot[slot][k]));
+      HIPCHK(hipEventRecord(ev_slot[slot][0], stream));
+    }
     // the whole pass is one graph launch; the stage times come from the device stamps
     if(gs)
     {
@@ -1203,6 +1232,24 @@ VkResult Queue::RunPass(const std::vector<Op *> &passes, bool fused_clear, ExecS
     else
       for(int stage = 0; stage < 4; stage++)
         HIPCHK(enqueue(stage));
+    if(deferring)
+    {
+      HIPCHK(hipEventRecord(ev_slot[slot][1], stream));
+      PassLaunch &pl = *deferred;
+      pl.launched = true;
+      pl.passes = passes;
+      pl.fused_clear = fused_clear;
+      pl.st = &st;
+      pl.sig = sig;
+      pl.cfg = cfg;
+      pl.pass_cap = pass_cap;
+      pl.ndraws = ndraws;
+      pl.tri_total = tri_total;
+      pl.tile_total = tile_total;
+      pl.slot = slot;
+      reset_guard.done = true;
+      return VK_SUCCESS;
+    }
     if(!ev_end)
       HIPCHK(hipEventCreate(&ev_end));
     HIPCHK(hipEventRecord(ev_end, stream));
@@ -1210,6 +1257,8 @@ VkResult Queue::RunPass(const std::vector<Op *> &passes, bool fused_clear, ExecS
     total_pairs = (uint32_t)h_out[0];
     if(total_pairs <= pass_cap)
       break;
+    // the launch set the overflow flag; it is cleared before the pass runs again
+    HIPCHK(va_fill_u32_async(poison, 1, 0u, stream));
     if(overflow_retries++ > 0)
     {
       LogError("bin capacity did not converge (%u pairs)", total_pairs);
@@ -1217,14 +1266,26 @@ VkResult Queue::RunPass(const std::vector<Op *> &passes, bool fused_clear, ExecS
     }
     pass_cap = total_pairs + total_pairs / 4 + 4096;
     pair_cap = std::max(pair_cap, pass_cap);
+    shape_cap = pass_cap;
   }
+  drained = true;  // everything enqueued so far has completed, ev_end marks its end
+  reset_guard.done = true;
+  PassDone((const uint64_t *)h_pinned + 16 * slot, total_pairs, sig, cfg, tuning, ndraws, tri_total, tile_total, st);
+  return VK_SUCCESS;
+}
+
+void Queue::PassDone(const uint64_t *h_out, uint32_t total_pairs, uint64_t sig, const TileConfig &cfg, bool tuning,
+                     uint32_t ndraws, uint32_t tri_total, uint32_t tile_total, ExecStats &st)
+{
   {
+    // later passes of this shape launch with the pairs it needed plus an eighth
     const uint64_t want = ((uint64_t)total_pairs + total_pairs / 8 + 4096 + 0xffffu) & ~(uint64_t)0xffffu;
-    shape_cap = (uint32_t)std::min<uint64_t>(std::max<uint64_t>(want, 1u << 16), 0xffff0000u);
+    if(shape_caps.size() > 4096)
+      shape_caps.clear();
+    shape_caps[sig * 8u + cfg.tile_shift] = (uint32_t)std::min<uint64_t>(std::max<uint64_t>(want, 1u << 16), 0xffff0000u);
   }
   VaCounters c;
   memset(&c, 0, sizeof(c));
-  const uint64_t *h_out = (const uint64_t *)h_pinned;
   memcpy(&c.tris_in, h_out + 2, 8 * sizeof(uint64_t));  // tris_in .. shaded_dwrite
   c.draws = ndraws;
   st.last_cfg = cfg;
@@ -1244,9 +1305,6 @@ VkResult Queue::RunPass(const std::vector<Op *> &passes, bool fused_clear, ExecS
     else
       LogError("stage timing unavailable (device clock stamps out of order)");
   }
-
-  drained = true;  // everything enqueued so far has completed, ev_end marks its end
-  reset_guard.done = true;
   st.raster_launches++;
   st.total_tris += tri_total;
   st.total_pairs += total_pairs;
@@ -1260,7 +1318,6 @@ VkResult Queue::RunPass(const std::vector<Op *> &passes, bool fused_clear, ExecS
   st.ctr.depth_passed += c.depth_passed;
   st.ctr.shaded_blend += c.shaded_blend;
   st.ctr.shaded_dwrite += c.shaded_dwrite;
-  return VK_SUCCESS;
 }
 
 void Queue::SignalFence(VkFence f, VkResult r)
@@ -1273,6 +1330,71 @@ void Queue::SignalFence(VkFence f, VkResult r)
   f->cv.notify_all();
 }
 
+VkResult Queue::CompleteOldest(std::vector<int> &open, std::vector<VkResult> &sres)
+{
+  PassLaunch p = std::move(inflight.front());
+  inflight.pop_front();
+  auto fail = [&](VkResult r) {
+    // nothing launched can be trusted: the pass and those behind it end without results
+    (void)hipStreamSynchronize(stream);
+    (void)hipGetLastError();
+    auto drop = [&](const PassLaunch &q) {
+      open[q.sub]--;
+      if(sres[q.sub] == VK_SUCCESS)
+        sres[q.sub] = r;
+    };
+    drop(p);
+    for(; !inflight.empty(); inflight.pop_front())
+      drop(inflight.front());
+    return r;
+  };
+  hipError_t e = hipEventSynchronize(ev_slot[p.slot][1]);
+  if(e != hipSuccess)
+    return fail(HipToVk(e));
+  float ms = 0.0f;
+  if(hipEventElapsedTime(&ms, ev_slot[p.slot][0], ev_slot[p.slot][1]) != hipSuccess)
+  {
+    ms = 0.0f;
+    (void)hipGetLastError();
+  }
+  p.st->gpu_ms_total += ms;
+  const uint64_t *h_out = (const uint64_t *)h_pinned + 16 * p.slot;
+  const uint32_t total = (uint32_t)h_out[0];
+  if(total <= p.pass_cap)
+  {
+    PassDone(h_out, total, p.sig, p.cfg, false, p.ndraws, p.tri_total, p.tile_total, *p.st);
+    open[p.sub]--;
+    return VK_SUCCESS;
+  }
+  // Its pairs overflowed: its raster wrote nothing, and every pass launched behind it saw the
+  // overflow flag and wrote nothing either. Once those have ended, the flag is cleared, the
+  // pass's shape gets the capacity it needed, and all of them run again in launch order.
+  std::deque<PassLaunch> redo;
+  redo.push_back(std::move(p));
+  while(!inflight.empty())
+  {
+    redo.push_back(std::move(inflight.front()));
+    inflight.pop_front();
+  }
+  e = hipStreamSynchronize(stream);
+  if(e == hipSuccess)
+    e = va_fill_u32_async((uint32_t *)d_stamps + 34, 1, 0u, stream);
+  const PassLaunch &f = redo.front();
+  const uint32_t need = (uint32_t)std::min<uint64_t>((uint64_t)total + total / 4 + 4096, 0xffff0000u);
+  pair_cap = std::max(pair_cap, need);
+  shape_caps[f.sig * 8u + f.cfg.tile_shift] = need;
+  VkResult r = e == hipSuccess ? VK_SUCCESS : HipToVk(e);
+  for(PassLaunch &q : redo)
+  {
+    if(r == VK_SUCCESS)
+      r = RunPass(q.passes, q.fused_clear, *q.st, nullptr);
+    if(r != VK_SUCCESS && sres[q.sub] == VK_SUCCESS)
+      sres[q.sub] = r;
+    open[q.sub]--;
+  }
+  return r;
+}
+
 VkResult Queue::Execute(std::vector<Submission> &subs)
 {
   std::lock_guard<std::mutex> lk(submit_m);
@@ -1283,9 +1405,90 @@ VkResult Queue::Execute(std::vector<Submission> &subs)
     HIPCHK(hipEventCreate(&ev_begin));
   if(!ev_end)
     HIPCHK(hipEventCreate(&ev_end));
-  for(Submission &sub : subs)
+  // Asynchronous batches run consecutive batched passes back to back: each pass is launched
+  // before the one ahead of it has been waited for, and completed (results read, an overflow
+  // handled) while the next one runs. A submission made of such passes gets its stats and its
+  // fence once all of them are complete; every other submission first waits for them.
+  const bool pipelined = async && mode == ExecMode::Batched;
+  std::vector<ExecStats> stats(subs.size());
+  std::vector<int> open(subs.size(), 0);          // passes launched and not complete, per submission
+  std::vector<VkResult> sres(subs.size(), VK_SUCCESS);
+  std::vector<char> launched(subs.size(), 0);     // every operation of the submission launched
+  size_t next_final = 0;
+  auto finalize = [&](size_t i) {
+    last_stats = stats[i];
+    AccumulateStats(accum_stats, stats[i]);
+    AccumulateStats(frame_stats, stats[i]);
+    SignalFence(subs[i].fence, sres[i]);
+  };
+  // completes passes in flight (all of them, or until one is left) and finalizes, in order, the
+  // submissions whose passes are all complete
+  auto settle = [&](bool all) {
+    VkResult r = VK_SUCCESS;
+    while(!inflight.empty() && (all || inflight.size() > 1))
+    {
+      VkResult e = CompleteOldest(open, sres);
+      if(r == VK_SUCCESS)
+        r = e;
+    }
+    while(next_final < subs.size() && launched[next_final] && open[next_final] == 0)
+      finalize(next_final++);
+    return r;
+  };
+  for(size_t i = 0; i < subs.size(); i++)
   {
-    ExecStats st;
+    Submission &sub = subs[i];
+    ExecStats &st = stats[i];
+    bool passes_only = pipelined && sub.downloads.empty();
+    for(const Op &op : sub.ops)
+      passes_only = passes_only && op.kind == Op::Pass;
+    if(passes_only)
+    {
+      for(VkDeviceMemory_T *m : sub.uploads)
+      {
+        hipError_t e = res == VK_SUCCESS ? hipMemcpyAsync(m->dev, m->host, m->size, hipMemcpyHostToDevice, stream)
+                                         : hipSuccess;
+        if(e != hipSuccess)
+          res = HipToVk(e);
+      }
+      for(Op &op : sub.ops)
+      {
+        if(res != VK_SUCCESS)
+          break;
+        PassLaunch pl;
+        pl.sub = i;
+        res = RunPass({&op}, true, st, &pl);
+        if(res == VK_SUCCESS && pl.drain_first)
+        {
+          res = settle(true);
+          pl = PassLaunch();
+          pl.sub = i;
+          if(res == VK_SUCCESS)
+            res = RunPass({&op}, true, st, &pl);
+        }
+        if(res == VK_SUCCESS && pl.launched)
+        {
+          open[i]++;
+          inflight.push_back(std::move(pl));
+          // one pass runs, at most one more waits behind it
+          VkResult e = settle(false);
+          if(res == VK_SUCCESS)
+            res = e;
+        }
+      }
+      if(sres[i] == VK_SUCCESS)
+        sres[i] = res;
+      launched[i] = 1;
+      VkResult e = settle(false);
+      if(res == VK_SUCCESS)
+        res = e;
+      continue;
+    }
+    {
+      VkResult e = settle(true);
+      if(res == VK_SUCCESS)
+        res = e;
+    }
     for(VkDeviceMemory_T *m : sub.uploads)
       HIPCHK(hipMemcpyAsync(m->dev, m->host, m->size, hipMemcpyHostToDevice, stream));
     HIPCHK(hipEventRecord(ev_begin, stream));
@@ -1302,12 +1505,12 @@ VkResult Queue::Execute(std::vector<Submission> &subs)
       else
       {
         res = ClearFrame(op.frame);
-        for(size_t i = 0; i < op.draws.size() && res == VK_SUCCESS; i++)
+        for(size_t k = 0; k < op.draws.size() && res == VK_SUCCESS; k++)
         {
           Op one;
           one.kind = Op::Pass;
           one.frame = op.frame;
-          one.draws.assign(1, op.draws[i]);
+          one.draws.assign(1, op.draws[k]);
           res = RunPass({&one}, false, st);
         }
       }
@@ -1335,11 +1538,13 @@ VkResult Queue::Execute(std::vector<Submission> &subs)
       (void)hipGetLastError();
     }
     st.gpu_ms_total = ms;
-    last_stats = st;
-    AccumulateStats(accum_stats, st);
-    AccumulateStats(frame_stats, st);
-    SignalFence(sub.fence, res);
+    sres[i] = res;
+    launched[i] = 1;
+    finalize(next_final++);
   }
+  VkResult e = settle(true);
+  if(res == VK_SUCCESS)
+    res = e;
   return res;
 }
 

@@ -418,7 +418,32 @@ public:
 
 private:
   VkResult Execute(std::vector<Submission> &subs);
-  VkResult RunPass(const std::vector<Op *> &passes, bool fused_clear, ExecStats &st);
+  // A batched pass launched without waiting for its end: asynchronous batches queue the next
+  // pass behind it and complete it afterwards, so the device does not idle between passes.
+  struct PassLaunch
+  {
+    bool launched = false;     // set by RunPass when it returned without waiting
+    bool drain_first = false;  // set by RunPass when the pass must wait for the ones in flight
+    std::vector<Op *> passes;
+    bool fused_clear = false;
+    ExecStats *st = nullptr;
+    size_t sub = 0;  // submission of the batch it belongs to
+    uint64_t sig = 0;
+    TileConfig cfg;
+    uint32_t pass_cap = 0, ndraws = 0, tri_total = 0, tile_total = 0;
+    int slot = 0;  // output words in h_pinned and events of the launch
+  };
+  // deferred == nullptr: the pass completes before RunPass returns; otherwise RunPass may
+  // return right after the launch (deferred->launched) and CompleteOldest finishes it
+  VkResult RunPass(const std::vector<Op *> &passes, bool fused_clear, ExecStats &st,
+                   PassLaunch *deferred = nullptr);
+  void PassDone(const uint64_t *h_out, uint32_t total_pairs, uint64_t sig, const TileConfig &cfg, bool tuning,
+                uint32_t ndraws, uint32_t tri_total, uint32_t tile_total, ExecStats &st);
+  // waits for the oldest launched pass and takes its results; a pass whose pairs overflowed is
+  // run again after every pass queued behind it has ended, and those are run again in order
+  VkResult CompleteOldest(std::vector<int> &open, std::vector<VkResult> &sres);
+  std::deque<PassLaunch> inflight;  // launch order, at most two
+  hipEvent_t ev_slot[2][2] = {};    // per output slot: before and after the pass's launch
   uint64_t PassSignature(const std::vector<Op *> &passes) const;
   TileConfig ChooseConfig(uint64_t sig, bool *tuning);
   void RecordTuning(uint64_t sig, const TileConfig &cfg, double ms);
@@ -468,6 +493,7 @@ private:
   void *up_frames_ptr = nullptr, *up_draws_ptr = nullptr, *up_blocks_ptr = nullptr;
   uint64_t up_version = 0;      // PassTables::version the device buffers hold (0 = compare)
   void *stamps_zeroed = nullptr;  // d_stamps allocation whose accumulators were zeroed
+  void *poison_zeroed = nullptr;  // d_stamps allocation whose overflow flag was zeroed
   uint64_t table_versions = 0;  // versions handed out to rebuilt reusable tables
   // device tables of one pass, built on the host
   struct PassTables

@@ -460,11 +460,14 @@ VA_INLINE u32 va_nth_bit(u64 m, u32 k)
 // descriptors, and each thread takes pairs of the group by stride, finding its primitive by a
 // binary search over the G + 1 offsets. Pairs at or beyond cap are not written (the host sees
 // the total and grows the buffers). Workgroup 0 stamps the start of binning.
+// *poison stays set from a pass whose pairs overflowed until the host has seen it: a pass queued
+// behind that one before then reports all ones as its total, so its raster writes nothing
+// either and the host runs both again, in order.
 #define VA_EMIT_G 1024u
 extern "C" __global__ __launch_bounds__(VA_EMIT_G) void va_emit(
     const u32 *__restrict__ block_first, const u32 *__restrict__ block_pairs, u32 nblocks,
     const uint4 *__restrict__ emit_info, const u32 *__restrict__ tile_count, u32 cap, u32 *__restrict__ keys,
-    u32 *__restrict__ vals, u32 *__restrict__ pair_total, u64 *__restrict__ stamps)
+    u32 *__restrict__ vals, u32 *__restrict__ pair_total, u64 *__restrict__ stamps, u32 *__restrict__ poison)
 {
   static_assert(VA_EMIT_G == 4u * VA_SETUP_PRIMS, "four setup blocks per group");
   __shared__ u32 s_off[VA_EMIT_G + 1u];
@@ -517,7 +520,12 @@ extern "C" __global__ __launch_bounds__(VA_EMIT_G) void va_emit(
   if(blockIdx.x == 0u)
   {
     if(tid == 0u)
-      *pair_total = base;
+    {
+      const bool over = base > cap;
+      if(over)
+        *poison = 1u;
+      *pair_total = over || *poison == 0u ? base : 0xffffffffu;
+    }
     base = 0u;
   }
   s_off[tid] = off + ex;
