@@ -8,10 +8,10 @@ triangles (about 1.85M), per-draw MVP, 4 mipmapped 512x512 textures sampled tril
 even draws opaque (LESS), odd draws alpha-blended with depth writes off. Inputs (vertex, index,
 texture data) are resident in HBM; the target stays in HBM.
 
-A step is one vkQueueSubmit of the recorded render pass (all 256 draws). value = covered fragments
-per second in the batched path with the K steps submitted back to back (asynchronous submit, one
-wait at the end, timed on the host between synchronisations); sync_submit_gfrag_s and the p50/p99
-frame latency come from K more steps submitted synchronously, one at a time. The plain
+A step is one synchronous vkQueueSubmit of the recorded render pass (all 256 draws), timed on the
+host between synchronisations. value = covered fragments per second in the batched path; the
+p50/p99 frame latency comes from the same steps, and back_to_back_gfrag_s from at least 100 more
+frames submitted asynchronously with one wait at the end. The plain
 per-draw path (setup, bin, raster and a host wait per draw, as the reference's DrawTriangles) is
 timed beside it, and the CPU baseline is the oracle's reference-structured threaded mode on a
 bounded sample of the draws. The other single-GPU workloads (1000 random triangles at 256x256,
@@ -126,22 +126,15 @@ def measure(ctx, scene, texel_bytes, steps, warmup, plain_steps, tile=0, graphs=
     for _ in range(warmup):
         r.render()
     one = ctx.stats(0)
-    # value: `steps` submissions back to back (asynchronous submit, one wait at the end); the
-    # queue launches each pass behind the previous one without a host round trip in between
     ctx.icd.vh_reset_stats(ctx.queue)
-    ctx.set_mode(batched=True, async_submit=True)
+    lat = []
     device_sync()
     t0 = time.perf_counter()
     for _ in range(steps):
-        ctx.icd.check(ctx.submit([r.cb]), "vkQueueSubmit")
-    ctx.icd.check(ctx.icd.vkQueueWaitIdle(ctx.queue), "vkQueueWaitIdle")
+        lat.append(r.render())
     device_sync()
     elapsed = time.perf_counter() - t0
-    ctx.set_mode(batched=True, async_submit=False)
     acc = ctx.stats(1)
-    # frame latency: `steps` synchronous submits (each returns when its frame is complete, as the
-    # reference's vkQueueSubmit), each timed on its own
-    lat = [r.render() for _ in range(steps)]
     frags = one["pixels_written"]
     raster_ms = acc["raster_ms"] / max(acc["raster_launches"], 1)
     color_bytes = 8 if scene.color_bpp == 8 else 4
@@ -151,9 +144,23 @@ def measure(ctx, scene, texel_bytes, steps, warmup, plain_steps, tile=0, graphs=
            "setup_ms": acc["setup_ms"] / max(steps, 1), "bin_ms": acc["bin_ms"] / max(steps, 1),
            "fragments": frags, "shaded": one["depth_passed"], "stats": one, "latencies": lat, "trials": trials,
            "graph_replays": acc["graph_launches"], "algorithmic_bytes": b_raster,
-           "sync_gfragments_s": frags * steps / sum(lat) / 1e9,
            "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                         "frac": round(achieved / HBM_PEAK_GBS, 4)}}
+    # back-to-back submissions (asynchronous submit, one wait at the end): the queue launches each
+    # pass behind the one still running. A few such frames first start the queue's worker thread
+    # and build the second output slot's graph; the rate is then taken over at least 100 frames,
+    # since the first and last frames of a burst run without a pass beside them.
+    ctx.set_mode(batched=True, async_submit=True)
+    for _ in range(4):
+        ctx.icd.check(ctx.submit([r.cb]), "vkQueueSubmit")
+    ctx.icd.check(ctx.icd.vkQueueWaitIdle(ctx.queue), "vkQueueWaitIdle")
+    n_b2b = max(steps, 100)
+    t0 = time.perf_counter()
+    for _ in range(n_b2b):
+        ctx.icd.check(ctx.submit([r.cb]), "vkQueueSubmit")
+    ctx.icd.check(ctx.icd.vkQueueWaitIdle(ctx.queue), "vkQueueWaitIdle")
+    out["back_to_back_gfragments_s"] = frags * n_b2b / (time.perf_counter() - t0) / 1e9
+    ctx.set_mode(batched=True, async_submit=False)
     if plain_steps:
         ctx.set_mode(batched=False)
         r.render()
@@ -169,7 +176,7 @@ def measure(ctx, scene, texel_bytes, steps, warmup, plain_steps, tile=0, graphs=
 
 def config_line(m, cpu):
     line = {"batched_gfragments_s": round(m["gfragments_s"], 4), "batched_ms_per_frame": round(m["ms_per_frame"], 4),
-            "batched_sync_submit_gfragments_s": round(m["sync_gfragments_s"], 4),
+            "batched_back_to_back_gfragments_s": round(m["back_to_back_gfragments_s"], 4),
             "raster_ms": round(m["raster_ms"], 4), "fragments_per_frame": m["fragments"],
             "shaded_per_frame": m["shaded"], "tile": m["stats"]["tile_size"], "chunk": m["stats"]["chunk"],
             "roofline": m["roofline"], "cpu_baseline": cpu}
@@ -359,7 +366,7 @@ def main():
                    "mode": "batched", "tile": tile, "chunk": one["chunk"], "persistent": bool(one["persistent"]),
                    "launch_shape": "autotuned" if args.tile == 0 else "fixed", "tuner_trial_frames": m["trials"],
                    "fragments_per_frame": m["fragments"], "shaded_per_frame": m["shaded"]},
-        "sync_submit_gfrag_s": round(m["sync_gfragments_s"], 4),
+        "back_to_back_gfrag_s": round(m["back_to_back_gfragments_s"], 4),
         "plain_per_draw_gfrag_s": round(plain, 4) if plain else None,
         "speedup_vs_plain": round(m["gfragments_s"] / plain, 2) if plain else None,
         "frame_latency_ms": {"p50": round(statistics.median(lat_ms), 3),
