@@ -824,14 +824,12 @@ VA_INLINE u64 va_transpose64(u64 x, u32 lane)
   return x;
 }
 
-// The same transpose when only lanes 0..31 hold rows (chunks of at most 32 records): lane i's row
-// is split into its pixel halves, lane i keeping pixels 0..31 and lane i + 32 taking pixels
-// 32..63, and each half of the wave transposes its 32x32 block in five 32-bit steps. Afterwards
-// bit i of lane j is the former element (i, j), for i < 32.
-VA_INLINE u32 va_transpose32x64(u64 x, u32 lane)
+// The same transpose for at most 32 rows (chunks of at most 32 records), each row held in two
+// halves: v of lane i < 32 holds elements 0..31 of row i, v of lane i + 32 elements 32..63 of
+// row i. Each half of the wave transposes its 32x32 block in five 32-bit steps; afterwards bit i
+// of lane j is the former element (i, j), for i < 32.
+VA_INLINE u32 va_transpose32(u32 v, u32 lane)
 {
-  const u32 hi = (u32)__shfl_xor((int)(u32)(x >> 32), 32);
-  u32 v = lane < 32u ? (u32)x : hi;
   const u32 masks[5] = {0x0000FFFFu, 0x00FF00FFu, 0x0F0F0F0Fu, 0x33333333u, 0x55555555u};
 #pragma unroll
   for(int i = 0; i < 5; i++)
@@ -1026,11 +1024,78 @@ VA_INLINE void va_raster_tile(const VaDraw *__restrict__ draws, const VaFrame *_
       stg.load_recs(rec, chunk_len(c + 1u));
       stg.load_tris(bins, beg, chunk_len(c + 2u), c + 2u);
 
-      // 1. Coverage, triangle-parallel where cheap: lane k tests triangle k against this wave's
-      //    block when the bbox overlap is at most VA_SMALL pixels (same integer arithmetic as the
-      //    pixel-parallel test); larger overlaps are tested pixel-parallel, one ballot per
-      //    triangle. Either way lane k ends up holding row k of the coverage matrix
-      //    (bit p = pixel p covered by triangle k).
+      // 1. Coverage, triangle-parallel where cheap: triangle k is tested against this wave's
+      //    block by its own lanes when the bbox overlap is at most VA_SMALL pixels (same integer
+      //    arithmetic as the pixel-parallel test); larger overlaps are tested pixel-parallel, one
+      //    ballot per triangle. The result is row k of the coverage matrix (bit p = pixel p
+      //    covered by triangle k).
+#if VA_CHUNK <= 32
+      //    Lanes k and k + 32 share triangle k: lane k tests the block's even rows and lane k + 32
+      //    its odd rows, each stepping two rows at a time; then each takes from the other the rows
+      //    of its half of the block, so that lane k holds pixels 0..31 (rows 0..3) of row k and
+      //    lane k + 32 pixels 32..63 (rows 4..7), the layout va_transpose32 takes.
+      const u32 tk = lane & 31u, half = lane >> 5;
+      u64 own = 0ull;
+      bool big = false;
+      u32 fl = 0u;
+      if(tk < n)
+      {
+        const u32 *r = s_rec + tk * LS;
+        fl = r[VA_R_DRAW] >> VA_DRAW_BITS;
+        const u32 bmin = r[VA_R_BBMIN], bmax = r[VA_R_BBMAX];
+        const i32 x0 = max((i32)(bmin & 0xffffu), bx), y0 = max((i32)(bmin >> 16), by);
+        const i32 x1 = min(min((i32)(bmax & 0xffffu), bx + 8), (i32)W);
+        const i32 y1 = min(min((i32)(bmax >> 16), by + 8), (i32)H);
+        if(x0 < x1 && y0 < y1)
+        {
+          if((x1 - x0) * (y1 - y0) > VA_SMALL)
+            big = half == 0u;
+          else
+          {
+            // the edge functions at (x0, ys), ys the lane's first row, then stepped: x + 1 adds
+            // ACY to ux and -ABY to uy, y + 2 adds -2 ACX and 2 ABX (the same values modulo 2^32
+            // as evaluating each pixel)
+            const u32 ABX = r[VA_R_ABX], ABY = r[VA_R_ABY], ACX = r[VA_R_ACX], ACY = r[VA_R_ACY], AR = r[VA_R_AREA];
+            const i32 ys = y0 + (((y0 - by) ^ (i32)half) & 1);
+            const u32 PAx = r[VA_R_X0] - (u32)x0, PAy = r[VA_R_Y0] - (u32)ys;
+            u32 ux_row = ACX * PAy - ACY * PAx;
+            u32 uy_row = PAx * ABY - PAy * ABX;
+            for(i32 y = ys; y < y1; y += 2)
+            {
+              u32 ux = ux_row, uy = uy_row;
+              for(i32 x = x0; x < x1; x++)
+              {
+                if((i32)(AR - (ux + uy)) >= 0 && (i32)ux >= 0 && (i32)uy >= 0)
+                  own |= 1ull << (u32)((y - by) * 8 + (x - bx));
+                ux += ACY;
+                uy -= ABY;
+              }
+              ux_row -= ACX << 1;
+              uy_row += ABX << 1;
+            }
+          }
+        }
+      }
+      const u32 recv = (u32)__shfl_xor((int)(half ? (u32)own : (u32)(own >> 32)), 32);
+      u32 rows = (half ? (u32)(own >> 32) : (u32)own) | recv;
+      // records of draws that blend (bit k = record k)
+      const va_cmask blends = (va_cmask)__ballot((fl & VA_F_BLEND) != 0u);
+      for(u64 bigs = __ballot(big); bigs; bigs &= bigs - 1ull)
+      {
+        const u32 k = (u32)__builtin_ctzll(bigs);
+        const u32 *r = s_rec + k * LS;
+        const u32 bmin = r[VA_R_BBMIN], bmax = r[VA_R_BBMAX];
+        const bool cov = inside && va_covered(r, px, py) && px >= (i32)(bmin & 0xffffu) && px < (i32)(bmax & 0xffffu) &&
+                         py >= (i32)(bmin >> 16) && py < (i32)(bmax >> 16);
+        const u64 m = __ballot(cov);
+        if(tk == k)
+          rows = half ? (u32)(m >> 32) : (u32)m;
+      }
+      // 2. Transpose: lane p now holds the set of records (bit k) covering pixel p, so the
+      //    in-order loop below runs once per fragment of this pixel, not once per triangle.
+      const va_cmask cov = va_transpose32(rows, lane);
+#else
+      //    Lane k tests triangle k and ends up holding row k.
       u64 row_mask = 0ull;
       bool big = false;
       u32 fl = 0u;
@@ -1085,9 +1150,6 @@ VA_INLINE void va_raster_tile(const VaDraw *__restrict__ draws, const VaFrame *_
       }
       // 2. Transpose: lane p now holds the set of records (bit k) covering pixel p, so the
       //    in-order loop below runs once per fragment of this pixel, not once per triangle.
-#if VA_CHUNK <= 32
-      const va_cmask cov = va_transpose32x64(row_mask, lane);
-#else
       const va_cmask cov = va_transpose64(row_mask, lane);
 #endif
 
