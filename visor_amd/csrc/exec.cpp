@@ -1097,9 +1097,13 @@ VkResult Queue::RunPass(const std::vector<Op *> &passes, bool fused_clear, ExecS
           uint32_t *keys = (uint32_t *)d_keys, *vals = (uint32_t *)d_vals;
           uint4 *einfo = (uint4 *)d_einfo;
           uint32_t *block_first = bd + nblocks;
-          void *eargs[] = {&block_first, &block_pairs, (void *)&nblocks, &einfo,  &cnt, &cap,
-                           &keys,        &vals,        &total_dev,       &stamps, &poison};
-          VA_TRY(hipModuleLaunchKernel(mod->emit, (nblocks + 3) / 4, 1, 1, 1024, 1, 1, 0, stream, eargs, nullptr));
+          // groups of four setup blocks; a group is split over several workgroups when the pass's
+          // capacity allows more than 4 pairs per thread on average (few, large triangles)
+          const uint32_t groups = (nblocks + 3) / 4;
+          uint32_t split = (uint32_t)std::min<uint64_t>(32, std::max<uint64_t>(1, (uint64_t)cap / (4096ull * groups)));
+          void *eargs[] = {&block_first, &block_pairs, (void *)&nblocks, &einfo,  &cnt,   &cap,
+                           &keys,        &vals,        &total_dev,       &stamps, &poison, &split};
+          VA_TRY(hipModuleLaunchKernel(mod->emit, groups * split, 1, 1, 1024, 1, 1, 0, stream, eargs, nullptr));
           int alt = 0;
           VA_TRY(va_sort_pairs((uint32_t *)d_stemp, keys, vals, (uint32_t *)d_keys2, (uint32_t *)d_vals2, total_dev, cap,
                                BitsFor(tile_total), &alt, stream));

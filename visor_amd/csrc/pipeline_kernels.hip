@@ -458,8 +458,11 @@ VA_INLINE u32 va_nth_bit(u64 m, u32 k)
 // blocks before it (setup's block_pairs; workgroup 0 also sums them all into *pair_total); the
 // primitives' offsets inside the group are an exclusive scan in LDS, next to their emit
 // descriptors, and each thread takes pairs of the group by stride, finding its primitive by a
-// binary search over the G + 1 offsets. Pairs at or beyond cap are not written (the host sees
-// the total and grows the buffers). Workgroup 0 stamps the start of binning.
+// binary search over the G + 1 offsets. A group whose primitives have many pairs each (few large
+// triangles) is expanded by `split` workgroups: each repeats the group's offsets and takes every
+// split-th G-pair step of its pairs, so that a pass of a few thousand primitives still spreads its
+// pairs over the GPU (workgroup b: group b / split, part b % split). Pairs at or beyond cap are not
+// written (the host sees the total and grows the buffers). Workgroup 0 stamps the start of binning.
 // *poison stays set from a pass whose pairs overflowed until the host has seen it: a pass queued
 // behind that one before then reports all ones as its total, so its raster writes nothing
 // either and the host runs both again, in order.
@@ -467,7 +470,8 @@ VA_INLINE u32 va_nth_bit(u64 m, u32 k)
 extern "C" __global__ __launch_bounds__(VA_EMIT_G) void va_emit(
     const u32 *__restrict__ block_first, const u32 *__restrict__ block_pairs, u32 nblocks,
     const uint4 *__restrict__ emit_info, const u32 *__restrict__ tile_count, u32 cap, u32 *__restrict__ keys,
-    u32 *__restrict__ vals, u32 *__restrict__ pair_total, u64 *__restrict__ stamps, u32 *__restrict__ poison)
+    u32 *__restrict__ vals, u32 *__restrict__ pair_total, u64 *__restrict__ stamps, u32 *__restrict__ poison,
+    u32 split)
 {
   static_assert(VA_EMIT_G == 4u * VA_SETUP_PRIMS, "four setup blocks per group");
   __shared__ u32 s_off[VA_EMIT_G + 1u];
@@ -476,8 +480,9 @@ extern "C" __global__ __launch_bounds__(VA_EMIT_G) void va_emit(
   const u32 tid = threadIdx.x, lane = tid & 63u, w = tid >> 6;
   if(blockIdx.x == 0u && tid == 0u)
     stamps[1] = (u64)wall_clock64();
+  const u32 group = blockIdx.x / split, part = blockIdx.x - group * split;
   // block_first[b] = first primitive of setup block b, block_first[nblocks] = the total
-  const u32 b0 = blockIdx.x * 4u;
+  const u32 b0 = group * 4u;
   const u32 p0 = block_first[b0], p1 = block_first[min(b0 + 4u, nblocks)];
   // pairs of the blocks before this group (workgroup 0: of all blocks, the total)
   const u32 lim = blockIdx.x == 0u ? nblocks : b0;
@@ -533,7 +538,7 @@ extern "C" __global__ __launch_bounds__(VA_EMIT_G) void va_emit(
     s_off[VA_EMIT_G] = L;
   __syncthreads();
   const u32 qend = base < cap ? min(L, cap - base) : 0u;
-  for(u32 q = tid; q < qend; q += VA_EMIT_G)
+  for(u32 q = part * VA_EMIT_G + tid; q < qend; q += VA_EMIT_G * split)
   {
     // last i with s_off[i] <= q (the steps sum to G - 1, so i + step stays below G)
     u32 i = 0;
