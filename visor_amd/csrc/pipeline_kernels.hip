@@ -55,17 +55,20 @@ VA_INLINE u64 va_wave_sum(u64 v)
   return v;
 }
 
-VA_INLINE u32 va_wave_exclusive_sum(u32 v, u32 lane)
+// Exclusive prefix sum over the wave's 64 lanes with DPP moves only (no LDS-routed shuffles):
+// an inclusive scan inside each row of 16 lanes (row_shr by 1, 2, 4, 8; a lane whose source lies
+// before its row's start adds 0), then row_bcast:15 adds lane 15 of rows 0 and 2 to rows 1 and 3,
+// and row_bcast:31 adds lane 31 to rows 2 and 3.
+VA_INLINE u32 va_wave_exclusive_sum(u32 v)
 {
-  u32 incl = v;
-#pragma unroll
-  for(u32 o = 1u; o < 64u; o <<= 1)
-  {
-    const u32 t = (u32)__shfl_up((int)incl, o);
-    if(lane >= o)
-      incl += t;
-  }
-  return incl - v;
+  u32 x = v;
+  x += (u32)__builtin_amdgcn_update_dpp(0, (int)x, 0x111, 0xf, 0xf, false);  // row_shr:1
+  x += (u32)__builtin_amdgcn_update_dpp(0, (int)x, 0x112, 0xf, 0xf, false);  // row_shr:2
+  x += (u32)__builtin_amdgcn_update_dpp(0, (int)x, 0x114, 0xf, 0xf, false);  // row_shr:4
+  x += (u32)__builtin_amdgcn_update_dpp(0, (int)x, 0x118, 0xf, 0xf, false);  // row_shr:8
+  x += (u32)__builtin_amdgcn_update_dpp(0, (int)x, 0x142, 0xa, 0xf, false);  // row_bcast:15
+  x += (u32)__builtin_amdgcn_update_dpp(0, (int)x, 0x143, 0xc, 0xf, false);  // row_bcast:31
+  return x - v;
 }
 
 // Sums four counters over the workgroup and writes them as one row (every workgroup writes
@@ -505,7 +508,7 @@ extern "C" __global__ __launch_bounds__(VA_EMIT_G) void va_emit(
   const u32 c = g < p1 ? tile_count[g] : 0u;
   if(c != 0u)
     s_info[tid] = emit_info[g];
-  const u32 ex = va_wave_exclusive_sum(c, lane);
+  const u32 ex = va_wave_exclusive_sum(c);
   const u32 wpre = __ockl_wfred_add_u32(pre);
   if(lane == 63u)
   {
@@ -838,18 +841,22 @@ VA_INLINE u64 va_transpose64(u64 x, u32 lane)
 // The same transpose for at most 32 rows (chunks of at most 32 records), each row held in two
 // halves: v of lane i < 32 holds elements 0..31 of row i, v of lane i + 32 elements 32..63 of
 // row i. Each half of the wave transposes its 32x32 block in five 32-bit steps; afterwards bit i
-// of lane j is the former element (i, j), for i < 32.
+// of lane j is the former element (i, j), for i < 32. The partner lane (lane ^ s, s <= 16, inside
+// the same 32 lanes) is read with ds_swizzle in bit-mask mode (and 0x1f, xor s), which needs no
+// per-lane address.
 VA_INLINE u32 va_transpose32(u32 v, u32 lane)
 {
-  const u32 masks[5] = {0x0000FFFFu, 0x00FF00FFu, 0x0F0F0F0Fu, 0x33333333u, 0x55555555u};
-#pragma unroll
-  for(int i = 0; i < 5; i++)
-  {
-    const u32 s = 16u >> i;
-    const u32 m = masks[i];
-    const u32 y = (u32)__shfl_xor((int)v, (int)s);
-    v = (lane & s) ? ((v & ~m) | ((y >> s) & m)) : ((v & m) | ((y << s) & ~m));
+#define VA_T32_STEP(S, M)                                                                       \
+  {                                                                                             \
+    const u32 y = (u32)__builtin_amdgcn_ds_swizzle((int)v, 0x1f | ((S) << 10));                \
+    v = (lane & (S)) ? ((v & ~(M)) | ((y >> (S)) & (M))) : ((v & (M)) | ((y << (S)) & ~(M))); \
   }
+  VA_T32_STEP(16u, 0x0000FFFFu)
+  VA_T32_STEP(8u, 0x00FF00FFu)
+  VA_T32_STEP(4u, 0x0F0F0F0Fu)
+  VA_T32_STEP(2u, 0x33333333u)
+  VA_T32_STEP(1u, 0x55555555u)
+#undef VA_T32_STEP
   return v;
 }
 
@@ -1250,7 +1257,7 @@ VA_INLINE void va_raster_tile(const VaDraw *__restrict__ draws, const VaFrame *_
       //    to the job list, the wave shades the list 64 wide, then each pixel blends its own
       //    results of the round in order.
       const u32 cnt = va_cmask_pop(alive);
-      const u32 off = va_wave_exclusive_sum(cnt, lane);
+      const u32 off = va_wave_exclusive_sum(cnt);
       const u32 total = (u32)__builtin_amdgcn_readlane((int)(off + cnt), 63);
       va_cmask wrem = alive, brem = alive;  // write and blend cursors over this pixel's jobs
       u32 wnext = off, bnext = off;
