@@ -47,12 +47,13 @@ VA_INLINE u32 va_find_frame(const VaFrame *__restrict__ frames, u32 nframes, u32
   return lo;
 }
 
-VA_INLINE u64 va_wave_sum(u64 v)
+extern "C" __device__ __attribute__((const)) u32 __ockl_wfred_add_u32(u32);
+
+// The wave's sum of a 32-bit value per lane, exact: the 16-bit halves are summed separately
+// (a sum of 64 halves stays below 2^22) by the DPP wave reduction.
+VA_INLINE u64 va_wave_sum32(u32 v)
 {
-#pragma unroll
-  for(int o = 32; o > 0; o >>= 1)
-    v += __shfl_xor(v, o);
-  return v;
+  return (u64)__ockl_wfred_add_u32(v & 0xffffu) + ((u64)__ockl_wfred_add_u32(v >> 16) << 16);
 }
 
 // Exclusive prefix sum over the wave's 64 lanes with DPP moves only (no LDS-routed shuffles):
@@ -73,13 +74,10 @@ VA_INLINE u32 va_wave_exclusive_sum(u32 v)
 
 // Sums four counters over the workgroup and writes them as one row (every workgroup writes
 // its row, so the partials need no clearing).
-VA_INLINE void va_block_partials(u64 *row, u64 a, u64 b, u64 c, u64 e)
+VA_INLINE void va_block_partials(u64 *row, u32 na, u32 nb, u32 nc, u32 ne)
 {
   __shared__ u64 s_part[16][4];
-  a = va_wave_sum(a);
-  b = va_wave_sum(b);
-  c = va_wave_sum(c);
-  e = va_wave_sum(e);
+  const u64 a = va_wave_sum32(na), b = va_wave_sum32(nb), c = va_wave_sum32(nc), e = va_wave_sum32(ne);
   const u32 wave = threadIdx.x >> 6, nw = (blockDim.x + 63u) >> 6;
   if((threadIdx.x & 63u) == 0u)
   {
@@ -97,8 +95,6 @@ VA_INLINE void va_block_partials(u64 *row, u64 a, u64 b, u64 c, u64 e)
     row[threadIdx.x] = t;
   }
 }
-
-extern "C" __device__ __attribute__((const)) u32 __ockl_wfred_add_u32(u32);
 
 // va_setup's counters over the workgroup as one row (tris_in, tris_out, blocks, pixels_tested):
 // the two flags are counted by ballot, blocks and pixels_tested summed as 32-bit values across
