@@ -18,6 +18,10 @@
 #define VA_REC_HDR 17   /* header words of a triangle record, varyings follow */
 #define VA_REC_VIS 14   /* leading header words the raster's visibility pass reads */
 #define VA_SETUP_PRIMS 256u /* primitives (threads) per setup workgroup (512: no faster) */
+/* Counter rows per stage: workgroup b of setup (tile t of the raster) adds its four counters
+ * into row b % VA_COUNTER_ROWS (t % VA_COUNTER_ROWS) of its stage's rows, 4 u64 each; the finish
+ * kernel sums the rows and leaves them zero. */
+#define VA_COUNTER_ROWS 64u
 
 /* record header word indices */
 #define VA_R_X0 0

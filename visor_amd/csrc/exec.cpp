@@ -811,8 +811,8 @@ void Queue::RecordTuning(uint64_t sig, const TileConfig &cfg, double ms)
 
 VkResult Queue::RunPass(const std::vector<Op *> &passes, bool fused_clear, ExecStats &st, PassLaunch *deferred)
 {
-  // A pass that stops part way may leave the finish kernel's accumulators and ticket mid-count:
-  // every early return makes the next pass zero them again.
+  // A pass that stops part way may leave counts in the counter rows: every early return makes
+  // the next pass zero them again.
   struct ResetOnEarlyReturn
   {
     void *&zeroed;
@@ -822,7 +822,7 @@ VkResult Queue::RunPass(const std::vector<Op *> &passes, bool fused_clear, ExecS
       if(!done)
         zeroed = nullptr;
     }
-  } reset_guard{stamps_zeroed};
+  } reset_guard{rows_zeroed};
   // a single pass decoded from a recording reuses the tables built for it last time
   const bool reusable = passes.size() == 1 && passes[0]->key != 0;
   const bool have = reusable && tables_cache.key == passes[0]->key &&
@@ -964,16 +964,17 @@ VkResult Queue::RunPass(const std::vector<Op *> &passes, bool fused_clear, ExecS
      !Grow(&d_bpairs, c_bpairs, 4ull * (nblocks + 1)) || !Grow(&d_start, c_start, 4ull * (tile_total + 1)) ||
      !Grow(&d_einfo, c_einfo, 16ull * std::max(tri_total, 1u)) || !Grow(&d_stamps, c_stamps, 256) ||
      !Grow(&d_blocks, c_blocks, 4ull * (2ull * nblocks + 1)) ||
-     !Grow(&d_part, c_part, 32ull * ((size_t)nblocks + tile_total + 1)))
+     !Grow(&d_part, c_part, 2ull * 32ull * VA_COUNTER_ROWS))
     return VK_ERROR_OUT_OF_DEVICE_MEMORY;
-  // d_stamps: stage stamps (u64 0-3), persistent tile counter (u32 8), finish accumulators (u64
-  // 8-15), ticket (u32 32) and overflow flag (u32 34); the accumulators, the ticket and the flag
-  // start at zero. The flag is zeroed only for a new allocation (afterwards the host clears it
+  // d_stamps: stage stamps (u64 0-3), persistent tile counter (u32 8) and overflow flag (u32 34).
+  // The flag starts at zero and is zeroed only for a new allocation (afterwards the host clears it
   // when it has seen the overflow): a pass launched behind one still running must see its flag.
-  if(d_stamps != stamps_zeroed)
+  // d_part: the counter rows of setup and raster, zero when a pass begins (the finish kernel
+  // leaves them zero; a pass that stopped part way has them zeroed again).
+  if(d_part != rows_zeroed)
   {
-    HIPCHK(va_fill_u32_async(d_stamps, 34, 0u, stream));
-    stamps_zeroed = d_stamps;
+    HIPCHK(va_fill_u32_async(d_part, 2 * 8 * VA_COUNTER_ROWS, 0u, stream));
+    rows_zeroed = d_part;
   }
   if(d_stamps != poison_zeroed)
   {
@@ -1054,8 +1055,8 @@ VkResult Queue::RunPass(const std::vector<Op *> &passes, bool fused_clear, ExecS
     // block_pairs: (tile, primitive) pairs per setup block, then the pair total
     uint32_t *rec = (uint32_t *)d_rec, *cnt = (uint32_t *)d_count, *block_pairs = (uint32_t *)d_bpairs;
     uint32_t *bd = (uint32_t *)d_blocks, *total_dev = block_pairs + nblocks;
-    // per-workgroup counter rows: setup's nblocks rows, then one raster row per tile
-    uint64_t *part = (uint64_t *)d_part, *part_r = part + 4ull * nblocks;
+    // counter rows: setup's VA_COUNTER_ROWS rows, then the raster's
+    uint64_t *part = (uint64_t *)d_part, *part_r = part + 4ull * VA_COUNTER_ROWS;
     uint64_t *h_out = (uint64_t *)h_pinned + 16 * slot;
     uint32_t *poison = (uint32_t *)d_stamps + 34;
     uint32_t cap = pass_cap;
@@ -1122,8 +1123,7 @@ VkResult Queue::RunPass(const std::vector<Op *> &passes, bool fused_clear, ExecS
         }
       }
       else if(stage == 3)
-        VA_TRY(va_finish_async(part, tri_total ? nblocks : 0u, part_r, tile_total, total_dev, stamps, stamps + 8,
-                               (uint32_t *)d_stamps + 32, h_out, stream));
+        VA_TRY(va_finish_async(part, VA_COUNTER_ROWS, total_dev, stamps, h_out, stream));
       else
       {
         // tile t's bin is [start[t], start[t + 1])

@@ -474,7 +474,7 @@ private:
   void *d_keys = nullptr, *d_vals = nullptr, *d_keys2 = nullptr, *d_vals2 = nullptr;
   void *d_start = nullptr, *d_blocks = nullptr;
   size_t c_blocks = 0;
-  void *d_part = nullptr;  // per-workgroup counter partials
+  void *d_part = nullptr;  // counter rows of setup and raster (VA_COUNTER_ROWS each)
   size_t c_part = 0;
   void *d_stemp = nullptr;  // radix sort histograms
   size_t c_stemp = 0;
@@ -492,7 +492,7 @@ private:
   std::vector<uint32_t> up_blocks;
   void *up_frames_ptr = nullptr, *up_draws_ptr = nullptr, *up_blocks_ptr = nullptr;
   uint64_t up_version = 0;      // PassTables::version the device buffers hold (0 = compare)
-  void *stamps_zeroed = nullptr;  // d_stamps allocation whose accumulators were zeroed
+  void *rows_zeroed = nullptr;    // d_part allocation whose counter rows were zeroed
   void *poison_zeroed = nullptr;  // d_stamps allocation whose overflow flag was zeroed
   uint64_t table_versions = 0;  // versions handed out to rebuilt reusable tables
   // device tables of one pass, built on the host

@@ -72,8 +72,15 @@ VA_INLINE u32 va_wave_exclusive_sum(u32 v)
   return x - v;
 }
 
-// Sums four counters over the workgroup and writes them as one row (every workgroup writes
-// its row, so the partials need no clearing).
+// Adds the row's four values into a counter row (device-scope atomics; the rows are zero when
+// the pass begins and the finish kernel sums and clears them).
+VA_INLINE void va_row_add(u64 *row, u32 i, u64 t)
+{
+  if(t != 0ull)
+    atomicAdd((unsigned long long *)&row[i], (unsigned long long)t);
+}
+
+// Sums four counters over the workgroup and adds them into a counter row.
 VA_INLINE void va_block_partials(u64 *row, u32 na, u32 nb, u32 nc, u32 ne)
 {
   __shared__ u64 s_part[16][4];
@@ -92,13 +99,13 @@ VA_INLINE void va_block_partials(u64 *row, u32 na, u32 nb, u32 nc, u32 ne)
     u64 t = 0;
     for(u32 w = 0; w < nw; w++)
       t += s_part[w][threadIdx.x];
-    row[threadIdx.x] = t;
+    va_row_add(row, threadIdx.x, t);
   }
 }
 
-// va_setup's counters over the workgroup as one row (tris_in, tris_out, blocks, pixels_tested):
-// the two flags are counted by ballot, blocks and pixels_tested summed as 32-bit values across
-// the wave (pixels_tested in 16-bit halves), then across the waves in LDS. Exact while a
+// va_setup's counters over the workgroup, added into a counter row (tris_in, tris_out, blocks,
+// pixels_tested): the two flags are counted by ballot, blocks and pixels_tested summed as 32-bit
+// values across the wave (pixels_tested in 16-bit halves), then across the waves in LDS. Exact while a
 // primitive's blocks and tested pixels stay below 2^26 and 2^42, i.e. for targets of at most
 // 65536 pixels per side (the device allows 16384).
 VA_INLINE void va_setup_partials(u64 *row, bool fin, bool fout, u32 blocks, u64 tested)
@@ -123,7 +130,7 @@ VA_INLINE void va_setup_partials(u64 *row, bool fin, bool fout, u32 blocks, u64 
 #pragma unroll
     for(u32 w = 0; w < VA_SETUP_PRIMS / 64u; w++)
       t += s_part[w][threadIdx.x];
-    row[threadIdx.x] = t;
+    va_row_add(row, threadIdx.x, t);
   }
 }
 
@@ -422,7 +429,7 @@ extern "C" __global__ __launch_bounds__(VA_SETUP_PRIMS) void va_setup(
     s_pairs[threadIdx.x >> 6] = wave_pairs;
   // counters: one row of partial sums per workgroup (no contended atomics), columns in
   // VaCounters order from tris_in; its barrier also completes s_pairs
-  va_setup_partials(part + (u64)blockIdx.x * 4u, n_in != 0u, n_out != 0u, n_blocks, n_tested);
+  va_setup_partials(part + (u64)(blockIdx.x % VA_COUNTER_ROWS) * 4u, n_in != 0u, n_out != 0u, n_blocks, n_tested);
   if(threadIdx.x == 0u)
   {
     u32 t = 0;
@@ -919,13 +926,9 @@ VA_INLINE void va_raster_tile(const VaDraw *__restrict__ draws, const VaFrame *_
   const bool inside = (u32)px < W && (u32)py < H;
   const u32 beg = bin_start[tile], en = bin_end[tile];
   const bool fused = use_fused_clear != 0u && fr.clear_flags != 0u;
-  u64 *row = part + (u64)tile * 4u;
+  u64 *row = part + (u64)(tile % VA_COUNTER_ROWS) * 4u;
   if(beg == en && !fused)
-  {
-    if(threadIdx.x < 4u)
-      row[threadIdx.x] = 0ull;
     return;
-  }
 
   const u64 pix = (u64)(u32)py * W + (u32)px;
   const bool has_depth = fr.depth != 0ull;

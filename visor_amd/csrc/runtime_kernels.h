@@ -49,12 +49,11 @@ extern "C" {
 #endif
 hipError_t va_transcode_async(const uint8_t *src, uint8_t *dst, const VaTranscode *t, hipStream_t stream);
 hipError_t va_stamp_async(uint64_t *out, hipStream_t stream);
-/* Counter sums, pair total and stage stamps of a pass into host-visible `out` (14 words).
- * acc (8 words) and ticket must be zero before the first launch; every launch leaves them zero. */
-hipError_t va_finish_async(const uint64_t *setup_part, uint32_t setup_rows, const uint64_t *raster_part,
-                           uint32_t raster_rows, const uint32_t *pair_total, const uint64_t *stamps, uint64_t *acc,
-                           uint32_t *ticket, uint64_t *out, hipStream_t stream);
-hipError_t va_reduce_cols_async(const uint64_t *part, uint32_t rows, uint32_t cols, uint64_t *out, hipStream_t stream);
+/* Counter sums, pair total and stage stamps of a pass into host-visible `out` (14 words): rows
+ * holds nrows (at most 64) counter rows of 4 words for setup, then as many for the raster; the
+ * launch sums them and leaves them zero. */
+hipError_t va_finish_async(uint64_t *rows, uint32_t nrows, const uint32_t *pair_total, const uint64_t *stamps,
+                           uint64_t *out, hipStream_t stream);
 #ifdef __cplusplus
 }
 #endif

@@ -509,69 +509,34 @@ extern "C" hipError_t va_transcode_async(const uint8_t *src, uint8_t *dst, const
   return hipGetLastError();
 }
 
-// End of a pass: sums the setup rows and the raster rows (4 counters each, in VaCounters order
-// from tris_in) and writes them, the pair total and the stage stamps straight into host-visible
+// End of a pass: sums the counter rows of setup and raster (nrows rows of 4 u64 per stage, setup's
+// first; in VaCounters order from tris_in and from pixels_written), clears them for the next
+// pass, and writes the sums, the pair total and the stage stamps straight into host-visible
 // memory, so the host reads one block after the stream completes.
 // out: [0] pair total, [2..9] counter sums, [10..12] stamps 0-2, [13] this launch's start.
-// Each workgroup adds its sums into acc[0..7] and takes a ticket; the workgroup that takes the
-// last ticket publishes the totals and returns acc and the ticket to zero for the next launch
-// (they start at zero when the buffer is allocated). No workgroup waits for another.
-constexpr uint32_t kFinishBlocks = 64;
+constexpr uint32_t kFinishMaxRows = 64;
 
-__global__ __launch_bounds__(256) void va_finish_kernel(const uint64_t *__restrict__ setup_part, uint32_t setup_rows,
-                                                        const uint64_t *__restrict__ raster_part, uint32_t raster_rows,
+__global__ __launch_bounds__(256) void va_finish_kernel(uint64_t *__restrict__ rows, uint32_t nrows,
                                                         const uint32_t *__restrict__ pair_total,
-                                                        const uint64_t *__restrict__ stamps, uint64_t *acc,
-                                                        uint32_t *ticket, uint64_t *out)
+                                                        const uint64_t *__restrict__ stamps, uint64_t *out)
 {
-  __shared__ uint64_t s_sum[4][8];
-  __shared__ uint32_t s_last;
+  __shared__ uint64_t s_v[2 * kFinishMaxRows * 4];
   const uint64_t t_start = (uint64_t)wall_clock64();
-  uint64_t v[8];
-#pragma unroll
-  for(int c = 0; c < 8; c++)
-    v[c] = 0ull;
-  const uint32_t stride = gridDim.x * 256u;
-  for(uint32_t r = blockIdx.x * 256u + threadIdx.x; r < setup_rows; r += stride)
+  const uint32_t nw = 2u * nrows * 4u;
+  for(uint32_t i = threadIdx.x; i < nw; i += 256u)
   {
-#pragma unroll
-    for(int c = 0; c < 4; c++)
-      v[c] += setup_part[(uint64_t)r * 4u + c];
-  }
-  for(uint32_t r = blockIdx.x * 256u + threadIdx.x; r < raster_rows; r += stride)
-  {
-#pragma unroll
-    for(int c = 0; c < 4; c++)
-      v[4 + c] += raster_part[(uint64_t)r * 4u + c];
-  }
-#pragma unroll
-  for(int c = 0; c < 8; c++)
-    for(int o = 32; o > 0; o >>= 1)
-      v[c] += (uint64_t)__shfl_xor((long long)v[c], o);
-  const uint32_t wave = threadIdx.x >> 6;
-  if((threadIdx.x & 63u) == 0u)
-  {
-#pragma unroll
-    for(int c = 0; c < 8; c++)
-      s_sum[wave][c] = v[c];
+    s_v[i] = rows[i];
+    rows[i] = 0ull;
   }
   __syncthreads();
   if(threadIdx.x < 8u)
   {
-    const uint64_t t = s_sum[0][threadIdx.x] + s_sum[1][threadIdx.x] + s_sum[2][threadIdx.x] + s_sum[3][threadIdx.x];
-    if(t)
-      atomicAdd((unsigned long long *)&acc[threadIdx.x], (unsigned long long)t);
+    const uint32_t stage = threadIdx.x >> 2, c = threadIdx.x & 3u;
+    uint64_t t = 0;
+    for(uint32_t r = 0; r < nrows; r++)
+      t += s_v[(stage * nrows + r) * 4u + c];
+    out[2 + threadIdx.x] = t;
   }
-  __threadfence();
-  __syncthreads();
-  if(threadIdx.x == 0u)
-    s_last = atomicAdd(ticket, 1u) == gridDim.x - 1u ? 1u : 0u;
-  __syncthreads();
-  if(!s_last)
-    return;
-  __threadfence();
-  if(threadIdx.x < 8u)
-    out[2 + threadIdx.x] = (uint64_t)atomicExch((unsigned long long *)&acc[threadIdx.x], 0ull);
   if(threadIdx.x == 0u)
   {
     out[0] = *pair_total;
@@ -579,17 +544,16 @@ __global__ __launch_bounds__(256) void va_finish_kernel(const uint64_t *__restri
     out[11] = stamps[1];
     out[12] = stamps[2];
     out[13] = t_start;
-    atomicExch(ticket, 0u);
   }
 }
 
-extern "C" hipError_t va_finish_async(const uint64_t *setup_part, uint32_t setup_rows, const uint64_t *raster_part,
-                                      uint32_t raster_rows, const uint32_t *pair_total, const uint64_t *stamps,
-                                      uint64_t *acc, uint32_t *ticket, uint64_t *out, hipStream_t stream)
+extern "C" hipError_t va_finish_async(uint64_t *rows, uint32_t nrows, const uint32_t *pair_total,
+                                      const uint64_t *stamps, uint64_t *out, hipStream_t stream)
 {
+  if(nrows == 0u || nrows > kFinishMaxRows)
+    return hipErrorInvalidValue;
   (void)hipGetLastError();
-  hipLaunchKernelGGL(va_finish_kernel, dim3(kFinishBlocks), dim3(256), 0, stream, setup_part, setup_rows, raster_part,
-                     raster_rows, pair_total, stamps, acc, ticket, out);
+  hipLaunchKernelGGL(va_finish_kernel, dim3(1), dim3(256), 0, stream, rows, nrows, pair_total, stamps, out);
   return hipGetLastError();
 }
 
@@ -619,45 +583,3 @@ extern "C" hipError_t va_fill_u64_async(void *p, uint64_t count, uint32_t lo, ui
   return hipGetLastError();
 }
 
-// out[c] += sum over rows of part[row * 4 + c] (four columns). Each workgroup reduces a slice
-// of rows and adds its four totals atomically: a few dozen atomics instead of one per wave.
-__global__ __launch_bounds__(256) void va_reduce4_kernel(const uint64_t *part, uint32_t rows, uint64_t *out)
-{
-  __shared__ uint64_t s[4][256];
-  uint64_t t[4] = {0, 0, 0, 0};
-  for(uint32_t r = blockIdx.x * blockDim.x + threadIdx.x; r < rows; r += gridDim.x * blockDim.x)
-  {
-    const ulonglong2 *p = (const ulonglong2 *)(part + (uint64_t)r * 4u);
-    ulonglong2 a = p[0], b = p[1];
-    t[0] += a.x;
-    t[1] += a.y;
-    t[2] += b.x;
-    t[3] += b.y;
-  }
-  for(int c = 0; c < 4; c++)
-    s[c][threadIdx.x] = t[c];
-  __syncthreads();
-  for(uint32_t o = blockDim.x / 2; o > 0; o >>= 1)
-  {
-    if(threadIdx.x < o)
-      for(int c = 0; c < 4; c++)
-        s[c][threadIdx.x] += s[c][threadIdx.x + o];
-    __syncthreads();
-  }
-  if(threadIdx.x < 4)
-    atomicAdd((unsigned long long *)&out[threadIdx.x], (unsigned long long)s[threadIdx.x][0]);
-}
-
-// Adds the column sums of a rows x 4 partials array into out[0..3] (out must start at zero).
-extern "C" hipError_t va_reduce_cols_async(const uint64_t *part, uint32_t rows, uint32_t cols, uint64_t *out,
-                                           hipStream_t stream)
-{
-  if(cols != 4)
-    return hipErrorInvalidValue;
-  uint32_t blocks = (rows + 1023) / 1024;
-  if(blocks < 1) blocks = 1;
-  if(blocks > 64) blocks = 64;
-  (void)hipGetLastError();  // report this launch's error only
-  hipLaunchKernelGGL(va_reduce4_kernel, dim3(blocks), dim3(256), 0, stream, part, rows, out);
-  return hipGetLastError();
-}
