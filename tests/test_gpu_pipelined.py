@@ -88,17 +88,22 @@ def test_back_to_back_passes_equal_the_oracle(ctx):
 def test_back_to_back_overflow_runs_both_passes_again_in_order():
     """A queue whose bin capacity has not grown (max(2^20, 2 x triangles)): the first pass draws
     1000 large random triangles on 640x480 with 8x8 tiles, about 1.25M (tile, triangle) pairs, so
-    its bins overflow; the pass submitted right behind it loads the same target and draws the
-    other 1000. The first writes nothing, the second sees the overflow flag and writes nothing
-    either, and both run again in order: the target equals the oracle's, the first pass's pairs
-    exceed the first capacity, and each fragment is counted once."""
+    its bins overflow; a small pass on another, smaller target is submitted right behind it (its
+    pair count is far below the capacity, and the pairs past it are the first pass's), then the
+    pass that loads the first target and draws the other 1000 triangles. The first writes nothing,
+    the one behind it sees the overflow flag and writes nothing either, and both run again in
+    order: both targets equal the oracle's, the first pass's pairs exceed the first capacity, and
+    each fragment is counted once."""
     fresh = render.Context()
     s = scenes.random_triangles(ntris=2000, width=640, height=480, seed=9)
-    ref = refrast.render(s)
+    small = scenes.random_triangles(ntris=60, width=96, height=64, seed=10)
+    ref, ref_small = refrast.render(s), refrast.render(small)
     r, first, second = _two_passes(fresh, s)
-    acc, last = _submit_back_to_back(fresh, [first, second])
+    rs = render.SceneRenderer(fresh, small, host_visible_targets=False)
+    acc, last = _submit_back_to_back(fresh, [first, rs.record(), second])
     _check_target(fresh, r, ref)
+    _check_target(fresh, rs, ref_small)
     assert acc["bin_pairs"] - last["bin_pairs"] > 1 << 20
     for k in ("pixels_written", "depth_passed", "tris_in", "pixels_tested"):
-        assert acc[k] == ref["counters"][k], k
-    assert acc["raster_launches"] == 2
+        assert acc[k] == ref["counters"][k] + ref_small["counters"][k], k
+    assert acc["raster_launches"] == 3

@@ -581,8 +581,11 @@ extern "C" __global__ __launch_bounds__(256) void va_ranges(const u32 *__restric
     u32 lo = 1u, hi = 0u;  // no tile
     if(i <= n)
     {
-      lo = i == 0u ? 0u : keys[i - 1u] + 1u;
-      hi = i == n ? ntiles : keys[i];
+      // keys are below ntiles; the clamps keep every write inside start[0..ntiles] also when a
+      // pass's raster is to write nothing (pairs overflowed, or a pass ahead of it overflowed:
+      // then the count may reach past the pairs this pass wrote)
+      lo = i == 0u ? 0u : min(keys[i - 1u], ntiles) + 1u;
+      hi = i == n ? ntiles : min(keys[i], ntiles);
     }
     const bool big = hi >= lo && hi - lo >= 16u;
     if(!big)
