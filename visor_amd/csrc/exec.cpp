@@ -1268,9 +1268,12 @@ VkResult Queue::RunPass(const std::vector<Op *> &passes, bool fused_clear, ExecS
       LogError("bin capacity did not converge (%u pairs)", total_pairs);
       return VK_ERROR_OUT_OF_DEVICE_MEMORY;
     }
-    pass_cap = total_pairs + total_pairs / 4 + 4096;
-    pair_cap = std::max(pair_cap, pass_cap);
-    shape_cap = pass_cap;
+    if(total_pairs != 0xffffffffu)  // all ones: the launch only saw an earlier overflow's flag
+    {
+      pass_cap = (uint32_t)std::min<uint64_t>((uint64_t)total_pairs + total_pairs / 4 + 4096, 0xffff0000u);
+      pair_cap = std::max(pair_cap, pass_cap);
+      shape_cap = pass_cap;
+    }
   }
   drained = true;  // everything enqueued so far has completed, ev_end marks its end
   reset_guard.done = true;
@@ -1383,10 +1386,13 @@ VkResult Queue::CompleteOldest(std::vector<int> &open, std::vector<VkResult> &sr
   e = hipStreamSynchronize(stream);
   if(e == hipSuccess)
     e = va_fill_u32_async((uint32_t *)d_stamps + 34, 1, 0u, stream);
-  const PassLaunch &f = redo.front();
-  const uint32_t need = (uint32_t)std::min<uint64_t>((uint64_t)total + total / 4 + 4096, 0xffff0000u);
-  pair_cap = std::max(pair_cap, need);
-  shape_caps[f.sig * 8u + f.cfg.tile_shift] = need;
+  if(total != 0xffffffffu)  // all ones: the pass only saw another's overflow flag
+  {
+    const PassLaunch &f = redo.front();
+    const uint32_t need = (uint32_t)std::min<uint64_t>((uint64_t)total + total / 4 + 4096, 0xffff0000u);
+    pair_cap = std::max(pair_cap, need);
+    shape_caps[f.sig * 8u + f.cfg.tile_shift] = need;
+  }
   VkResult r = e == hipSuccess ? VK_SUCCESS : HipToVk(e);
   for(PassLaunch &q : redo)
   {
