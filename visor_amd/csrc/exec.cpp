@@ -103,10 +103,7 @@ Queue::~Queue()
     for(hipGraphExec_t x : g.exec)
       if(x)
         hipGraphExecDestroy(x);
-  if(ev_begin)
-    hipEventDestroy(ev_begin);
-  if(ev_end)
-    hipEventDestroy(ev_end);
+
   if(stream)
     hipStreamDestroy(stream);
 }
@@ -978,8 +975,10 @@ VkResult Queue::RunPass(const std::vector<Op *> &passes, bool fused_clear, ExecS
   }
   if(d_stamps != poison_zeroed)
   {
-    HIPCHK(va_fill_u32_async((uint32_t *)d_stamps + 34, 1, 0u, stream));
+    // the overflow flag (word 34) and the pass sequence counter (word 38)
+    HIPCHK(va_fill_u32_async((uint32_t *)d_stamps + 34, 6, 0u, stream));
     poison_zeroed = d_stamps;
+    launch_seq = 0;
   }
   if(!h_pinned)  // written by the finish kernel, read by the host after the stream completes
     HIPCHK(hipHostMalloc((void **)&h_pinned, 4096, hipHostMallocCoherent));
@@ -1059,6 +1058,7 @@ VkResult Queue::RunPass(const std::vector<Op *> &passes, bool fused_clear, ExecS
     uint64_t *part = (uint64_t *)d_part, *part_r = part + 4ull * VA_COUNTER_ROWS;
     uint64_t *h_out = (uint64_t *)h_pinned + 16 * slot;
     uint32_t *poison = (uint32_t *)d_stamps + 34;
+    uint32_t *seq_dev = (uint32_t *)d_stamps + 38;  // passes that reached the finish kernel
     uint32_t cap = pass_cap;
     // The device steps of the pass in four stages, kernels only, captured together into one
     // graph: 0 setup, 1 binning, 2 raster, 3 finish (counter sums, pair total and the stage
@@ -1123,7 +1123,7 @@ VkResult Queue::RunPass(const std::vector<Op *> &passes, bool fused_clear, ExecS
         }
       }
       else if(stage == 3)
-        VA_TRY(va_finish_async(part, VA_COUNTER_ROWS, total_dev, stamps, h_out, stream));
+        VA_TRY(va_finish_async(part, VA_COUNTER_ROWS, total_dev, stamps, h_out, seq_dev, stream));
       else
       {
         // tile t's bin is [start[t], start[t + 1])
@@ -1219,14 +1219,6 @@ VkResult Queue::RunPass(const std::vector<Op *> &passes, bool fused_clear, ExecS
         }
       }
     }
-    const bool deferring = defer;
-    if(deferring)
-    {
-      for(int k = 0; k < 2; k++)
-        if(!ev_slot[slot][k])
-          HIPCHK(hipEventCreate(&ev_slot[slot][k]));
-      HIPCHK(hipEventRecord(ev_slot[slot][0], stream));
-    }
     // the whole pass is one graph launch; the stage times come from the device stamps
     if(gs)
     {
@@ -1236,9 +1228,10 @@ VkResult Queue::RunPass(const std::vector<Op *> &passes, bool fused_clear, ExecS
     else
       for(int stage = 0; stage < 4; stage++)
         HIPCHK(enqueue(stage));
-    if(deferring)
+    // the finish kernel of this launch will publish this number in the slot's word 14
+    const uint32_t seq = ++launch_seq;
+    if(defer)
     {
-      HIPCHK(hipEventRecord(ev_slot[slot][1], stream));
       PassLaunch &pl = *deferred;
       pl.launched = true;
       pl.passes = passes;
@@ -1251,13 +1244,11 @@ VkResult Queue::RunPass(const std::vector<Op *> &passes, bool fused_clear, ExecS
       pl.tri_total = tri_total;
       pl.tile_total = tile_total;
       pl.slot = slot;
+      pl.seq = seq;
       reset_guard.done = true;
       return VK_SUCCESS;
     }
-    if(!ev_end)
-      HIPCHK(hipEventCreate(&ev_end));
-    HIPCHK(hipEventRecord(ev_end, stream));
-    HIPCHK(hipStreamSynchronize(stream));
+    HIPCHK(WaitPass(slot, seq));
     total_pairs = (uint32_t)h_out[0];
     if(total_pairs <= pass_cap)
       break;
@@ -1275,7 +1266,7 @@ VkResult Queue::RunPass(const std::vector<Op *> &passes, bool fused_clear, ExecS
       shape_cap = pass_cap;
     }
   }
-  drained = true;  // everything enqueued so far has completed, ev_end marks its end
+  drained = true;  // everything enqueued so far has completed
   reset_guard.done = true;
   PassDone((const uint64_t *)h_pinned + 16 * slot, total_pairs, sig, cfg, tuning, ndraws, tri_total, tile_total, st);
   return VK_SUCCESS;
@@ -1303,6 +1294,7 @@ void Queue::PassDone(const uint64_t *h_out, uint32_t total_pairs, uint64_t sig, 
     const double khz = (double)Device::Get().wall_khz;
     if(khz > 0.0 && s0 <= s1 && s1 <= s2 && s2 <= s3)
     {
+      st.gpu_ms_total += (double)(s3 - s0) / khz;
       st.setup_ms += (double)(s1 - s0) / khz;
       st.bin_ms += (double)(s2 - s1) / khz;
       st.raster_ms += (double)(s3 - s2) / khz;
@@ -1337,6 +1329,30 @@ void Queue::SignalFence(VkFence f, VkResult r)
   f->cv.notify_all();
 }
 
+hipError_t Queue::WaitPass(int slot, uint32_t seq)
+{
+  // The finish kernel writes the sequence number into fine-grained host memory after the rest of
+  // the slot: spin on it (a pass is a fraction of a millisecond), and block on the stream only
+  // when it takes longer than 50 ms.
+  volatile const uint64_t *w = (volatile const uint64_t *)h_pinned + 16 * slot + 14;
+  const double t0 = NowSeconds();
+  for(uint32_t spin = 1; (uint32_t)*w != seq; spin++)
+  {
+    if((spin & 1023u) == 0u && NowSeconds() - t0 > 0.05)
+    {
+      const hipError_t e = hipStreamSynchronize(stream);
+      if(e != hipSuccess)
+        return e;
+      return (uint32_t)*w == seq ? hipSuccess : hipErrorUnknown;
+    }
+#if defined(__x86_64__)
+    __builtin_ia32_pause();
+#endif
+  }
+  std::atomic_thread_fence(std::memory_order_acquire);
+  return hipSuccess;
+}
+
 VkResult Queue::CompleteOldest(std::vector<int> &open, std::vector<VkResult> &sres)
 {
   PassLaunch p = std::move(inflight.front());
@@ -1355,16 +1371,9 @@ VkResult Queue::CompleteOldest(std::vector<int> &open, std::vector<VkResult> &sr
       drop(inflight.front());
     return r;
   };
-  hipError_t e = hipEventSynchronize(ev_slot[p.slot][1]);
+  hipError_t e = WaitPass(p.slot, p.seq);
   if(e != hipSuccess)
     return fail(HipToVk(e));
-  float ms = 0.0f;
-  if(hipEventElapsedTime(&ms, ev_slot[p.slot][0], ev_slot[p.slot][1]) != hipSuccess)
-  {
-    ms = 0.0f;
-    (void)hipGetLastError();
-  }
-  p.st->gpu_ms_total += ms;
   const uint64_t *h_out = (const uint64_t *)h_pinned + 16 * p.slot;
   const uint32_t total = (uint32_t)h_out[0];
   if(total <= p.pass_cap)
@@ -1411,10 +1420,6 @@ VkResult Queue::Execute(std::vector<Submission> &subs)
   if(!stream)
     HIPCHK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
   VkResult res = VK_SUCCESS;
-  if(!ev_begin)
-    HIPCHK(hipEventCreate(&ev_begin));
-  if(!ev_end)
-    HIPCHK(hipEventCreate(&ev_end));
   // Asynchronous batches run consecutive batched passes back to back: each pass is launched
   // before the one ahead of it has been waited for, and completed (results read, an overflow
   // handled) while the next one runs. A submission made of such passes gets its stats and its
@@ -1501,7 +1506,6 @@ VkResult Queue::Execute(std::vector<Submission> &subs)
     }
     for(VkDeviceMemory_T *m : sub.uploads)
       HIPCHK(hipMemcpyAsync(m->dev, m->host, m->size, hipMemcpyHostToDevice, stream));
-    HIPCHK(hipEventRecord(ev_begin, stream));
     drained = false;
     for(Op &op : sub.ops)
     {
@@ -1528,8 +1532,6 @@ VkResult Queue::Execute(std::vector<Submission> &subs)
     // one synchronization per submission: none more when the last operation was a pass that
     // has already waited for its work and nothing needs copying back
     bool need_sync = !drained;
-    if(!drained)
-      HIPCHK(hipEventRecord(ev_end, stream));
     for(VkDeviceMemory_T *m : sub.downloads)
     {
       HIPCHK(hipMemcpyAsync(m->host, m->dev, m->size, hipMemcpyDeviceToHost, stream));
@@ -1541,13 +1543,6 @@ VkResult Queue::Execute(std::vector<Submission> &subs)
       if(e != hipSuccess && res == VK_SUCCESS)
         res = HipToVk(e);
     }
-    float ms = 0.0f;
-    if(hipEventElapsedTime(&ms, ev_begin, ev_end) != hipSuccess)
-    {
-      ms = 0.0f;
-      (void)hipGetLastError();
-    }
-    st.gpu_ms_total = ms;
     sres[i] = res;
     launched[i] = 1;
     finalize(next_final++);

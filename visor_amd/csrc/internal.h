@@ -350,8 +350,8 @@ struct TileConfig
 
 struct ExecStats
 {
-  double gpu_ms_total = 0.0;    // hipEvent time around the whole submission's device work
-  double raster_ms = 0.0;       // hipEvent time around the raster kernel(s)
+  double gpu_ms_total = 0.0;    // device time of the submission's passes (setup start to finish start)
+  double raster_ms = 0.0;       // raster start to finish start, device clock stamps
   double setup_ms = 0.0;
   double bin_ms = 0.0;
   uint64_t raster_launches = 0;
@@ -431,7 +431,8 @@ private:
     uint64_t sig = 0;
     TileConfig cfg;
     uint32_t pass_cap = 0, ndraws = 0, tri_total = 0, tile_total = 0;
-    int slot = 0;  // output words in h_pinned and events of the launch
+    int slot = 0;      // output words of the launch in h_pinned
+    uint32_t seq = 0;  // the number its finish kernel publishes in the slot's word 14
   };
   // deferred == nullptr: the pass completes before RunPass returns; otherwise RunPass may
   // return right after the launch (deferred->launched) and CompleteOldest finishes it
@@ -443,7 +444,10 @@ private:
   // run again after every pass queued behind it has ended, and those are run again in order
   VkResult CompleteOldest(std::vector<int> &open, std::vector<VkResult> &sres);
   std::deque<PassLaunch> inflight;  // launch order, at most two
-  hipEvent_t ev_slot[2][2] = {};    // per output slot: before and after the pass's launch
+  // passes launched on this queue since d_stamps was allocated (the finish kernel counts the same
+  // on the device); WaitPass waits until slot `slot` holds sequence number seq
+  uint32_t launch_seq = 0;
+  hipError_t WaitPass(int slot, uint32_t seq);
   uint64_t PassSignature(const std::vector<Op *> &passes) const;
   TileConfig ChooseConfig(uint64_t sig, bool *tuning);
   void RecordTuning(uint64_t sig, const TileConfig &cfg, double ms);
@@ -463,9 +467,7 @@ private:
 
   hipStream_t stream = nullptr;
   std::mutex submit_m;  // one submission executes at a time
-  // device time of a submission: ev_begin before its first operation, ev_end after its last;
-  // RunPass records ev_end before the synchronization it needs anyway and sets drained
-  hipEvent_t ev_begin = nullptr, ev_end = nullptr;
+  // set by RunPass when every operation enqueued so far has completed
   bool drained = false;
   // growable device scratch
   // d_count: tiles per primitive; d_bpairs: pairs per setup block, then the pair total; d_start:

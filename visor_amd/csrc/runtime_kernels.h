@@ -49,11 +49,12 @@ extern "C" {
 #endif
 hipError_t va_transcode_async(const uint8_t *src, uint8_t *dst, const VaTranscode *t, hipStream_t stream);
 hipError_t va_stamp_async(uint64_t *out, hipStream_t stream);
-/* Counter sums, pair total and stage stamps of a pass into host-visible `out` (14 words): rows
+/* Counter sums, pair total and stage stamps of a pass into host-visible `out` (15 words): rows
  * holds nrows (at most 64) counter rows of 4 words for setup, then as many for the raster; the
- * launch sums them and leaves them zero. */
+ * launch sums them and leaves them zero. Last, out[14] = ++*seq (the number of launches so far,
+ * *seq zero before the first), written after every other word of out is visible to the host. */
 hipError_t va_finish_async(uint64_t *rows, uint32_t nrows, const uint32_t *pair_total, const uint64_t *stamps,
-                           uint64_t *out, hipStream_t stream);
+                           uint64_t *out, uint32_t *seq, hipStream_t stream);
 #ifdef __cplusplus
 }
 #endif

@@ -512,13 +512,18 @@ extern "C" hipError_t va_transcode_async(const uint8_t *src, uint8_t *dst, const
 // End of a pass: sums the counter rows of setup and raster (nrows rows of 4 u64 per stage, setup's
 // first; in VaCounters order from tris_in and from pixels_written), clears them for the next
 // pass, and writes the sums, the pair total and the stage stamps straight into host-visible
-// memory, so the host reads one block after the stream completes.
-// out: [0] pair total, [2..9] counter sums, [10..12] stamps 0-2, [13] this launch's start.
+// memory, then the pass's sequence number: *seq counts the passes that reached this kernel, and
+// out[14] receives the new count once every other word of out is visible to the host, so the
+// host learns that the pass is complete by reading host memory (no event or stream wait, which
+// would make every pass end with a system-scope release).
+// out: [0] pair total, [2..9] counter sums, [10..12] stamps 0-2, [13] this launch's start,
+// [14] sequence number.
 constexpr uint32_t kFinishMaxRows = 64;
 
 __global__ __launch_bounds__(256) void va_finish_kernel(uint64_t *__restrict__ rows, uint32_t nrows,
                                                         const uint32_t *__restrict__ pair_total,
-                                                        const uint64_t *__restrict__ stamps, uint64_t *out)
+                                                        const uint64_t *__restrict__ stamps, uint64_t *out,
+                                                        uint32_t *__restrict__ seq)
 {
   __shared__ uint64_t s_v[2 * kFinishMaxRows * 4];
   const uint64_t t_start = (uint64_t)wall_clock64();
@@ -545,15 +550,24 @@ __global__ __launch_bounds__(256) void va_finish_kernel(uint64_t *__restrict__ r
     out[12] = stamps[2];
     out[13] = t_start;
   }
+  __threadfence_system();  // each writer's words of out, before the sequence number
+  __syncthreads();
+  if(threadIdx.x == 0u)
+  {
+    const uint32_t n = *seq + 1u;  // one workgroup per launch, launches in stream order
+    *seq = n;
+    __threadfence_system();
+    *(volatile uint64_t *)&out[14] = n;
+  }
 }
 
 extern "C" hipError_t va_finish_async(uint64_t *rows, uint32_t nrows, const uint32_t *pair_total,
-                                      const uint64_t *stamps, uint64_t *out, hipStream_t stream)
+                                      const uint64_t *stamps, uint64_t *out, uint32_t *seq, hipStream_t stream)
 {
   if(nrows == 0u || nrows > kFinishMaxRows)
     return hipErrorInvalidValue;
   (void)hipGetLastError();
-  hipLaunchKernelGGL(va_finish_kernel, dim3(1), dim3(256), 0, stream, rows, nrows, pair_total, stamps, out);
+  hipLaunchKernelGGL(va_finish_kernel, dim3(1), dim3(256), 0, stream, rows, nrows, pair_total, stamps, out, seq);
   return hipGetLastError();
 }
 
