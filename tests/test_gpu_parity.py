@@ -122,6 +122,25 @@ def test_stacked_bins_keep_api_order(ctx, ntris):
     assert out["stats"]["depth_passed"] == ref["counters"]["depth_passed"]
 
 
+@pytest.mark.parametrize("chunk", [32, 64])
+def test_pixel_parallel_coverage_path(monkeypatch, chunk):
+    """A queue whose small-triangle threshold is 16 pixels (VISOR_SMALL; the default tests every
+    triangle triangle-parallel) sends larger bbox overlaps to the pixel-parallel test with one
+    ballot per triangle: random large triangles and the 256-draw mix at reduced size equal the
+    oracle through that path too, for chunks of 32 (two lanes per triangle) and 64."""
+    monkeypatch.setenv("VISOR_SMALL", "16")
+    monkeypatch.setenv("VISOR_CHUNK", str(chunk))
+    q = render.Context()  # the thresholds are read when the queue is created
+    for s in (scenes.random_triangles(ntris=500, width=200, height=136, seed=21),
+              scenes.many_draws(ndraws=24, width=320, height=240, tex_size=64, max_tris=2048)):
+        ref = refrast.render(s)
+        r = render.SceneRenderer(q, s)
+        q.set_mode(batched=True, tile=8)
+        r.render()
+        _compare(s, {"color": r.read_color(), "depth": r.read_depth()}, ref)
+        assert q.stats(0)["chunk"] == chunk
+
+
 def test_bin_overflow_reruns_the_pass(ctx):
     """2000 large random triangles on 640x480 with 8x8 tiles make about 2.5M (tile, triangle)
     pairs, over twice the first bin capacity of a new queue (max(2^20, 2 x triangles)): the first

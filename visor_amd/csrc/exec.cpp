@@ -740,11 +740,12 @@ static int BitsFor(uint32_t n)
 // ---------------------------------------------------------------------------------------------
 // Autotuner. Every launch shape (tile size and its workgroup of (T/8)^2 waves, LDS chunk) gives
 // the same bytes, so a repeated pass shape is simply run with the next untried candidate and
-// timed by the stage events; after two rounds over the candidates the fastest is kept.
+// timed by the device stamps; after two rounds over the candidates the fastest is kept. The
+// small-triangle threshold (third field) is the queue's for every candidate.
 
 static const TileConfig kTuneCandidates[] = {
-    {3, 64, 16, 0}, {3, 32, 16, 0}, {4, 64, 16, 0}, {4, 32, 16, 0}, {5, 64, 16, 0},
-    {3, 32, 16, 1}, {4, 32, 16, 1},
+    {3, 64, 0, 0}, {3, 32, 0, 0}, {4, 64, 0, 0}, {4, 32, 0, 0}, {5, 64, 0, 0},
+    {3, 32, 0, 1}, {4, 32, 0, 1},
 };
 static const int kTuneRounds = 2;
 
@@ -780,7 +781,9 @@ TileConfig Queue::ChooseConfig(uint64_t sig, bool *tuning)
     return t.best;
   const size_t nc = sizeof(kTuneCandidates) / sizeof(kTuneCandidates[0]);
   *tuning = true;
-  return kTuneCandidates[t.runs % nc];
+  TileConfig c = kTuneCandidates[t.runs % nc];
+  c.small = tiles.small;  // the candidates differ in tile size, chunk and grid form only
+  return c;
 }
 
 void Queue::RecordTuning(uint64_t sig, const TileConfig &cfg, double ms)
@@ -801,6 +804,7 @@ void Queue::RecordTuning(uint64_t sig, const TileConfig &cfg, double ms)
       if(t.ms[k] < t.ms[b])
         b = k;
     t.best = kTuneCandidates[b];
+    t.best.small = tiles.small;
     t.done = true;
     (void)cfg;
   }

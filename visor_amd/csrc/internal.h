@@ -335,12 +335,13 @@ struct Submission
 
 // Launch shape of the raster stage. tile_shift selects the kernel at launch; chunk (bin
 // entries staged in LDS per round, 16..64) and small (largest bbox-in-block overlap tested
-// triangle-parallel) are compile-time parameters of the pipeline module.
+// triangle-parallel; 64, the whole 8x8 block, tests every triangle that way) are compile-time
+// parameters of the pipeline module.
 struct TileConfig
 {
   uint32_t tile_shift = 3;  // 3, 4, 5 -> 8, 16, 32 pixel tiles
   uint32_t chunk = 64;
-  uint32_t small = 16;
+  uint32_t small = 64;
   uint32_t persistent = 0;  // 1: one GPU-filling grid taking tiles from a counter
   bool operator==(const TileConfig &o) const
   {

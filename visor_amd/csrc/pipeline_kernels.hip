@@ -746,7 +746,7 @@ VA_INLINE void va_wave_sync()
 #define VA_CHUNK 64u       // bin entries staged in LDS per round (at most 64: one bit each)
 #endif
 #ifndef VA_SMALL
-#define VA_SMALL 16        // bbox-in-block overlaps up to this many pixels are tested per triangle
+#define VA_SMALL 64        // bbox-in-block overlaps up to this many pixels are tested per triangle
 #endif
 static_assert(VA_CHUNK <= 64u, "record sets are 64-bit masks");
 
