@@ -492,6 +492,14 @@ extern "C" __global__ __launch_bounds__(VA_EMIT_G) void va_emit(
   const u32 p0 = block_first[b0], p1 = block_first[min(b0 + 4u, nblocks)];
   // pairs of the blocks before this group (workgroup 0: of all blocks, the total)
   const u32 lim = blockIdx.x == 0u ? nblocks : b0;
+  // this thread's primitive, loaded before the prefix so that both latencies overlap (emit_info
+  // holds 16 bytes for every primitive, so the load is in bounds also for a culled one)
+  const u32 g = p0 + tid;
+  const bool in = g < p1;
+  const u32 c = in ? tile_count[g] : 0u;
+  uint4 info = make_uint4(0u, 0u, 0u, 0u);
+  if(in)
+    info = emit_info[g];
   u32 pre = 0;
   for(u32 bb = 0; bb < lim; bb += 8u * VA_EMIT_G)
   {
@@ -507,10 +515,8 @@ extern "C" __global__ __launch_bounds__(VA_EMIT_G) void va_emit(
     for(u32 k = 0; k < 8u; k++)
       pre += x[k];
   }
-  const u32 g = p0 + tid;
-  const u32 c = g < p1 ? tile_count[g] : 0u;
   if(c != 0u)
-    s_info[tid] = emit_info[g];
+    s_info[tid] = info;
   const u32 ex = va_wave_exclusive_sum(c);
   const u32 wpre = __ockl_wfred_add_u32(pre);
   if(lane == 63u)
