@@ -208,13 +208,19 @@ __global__ __launch_bounds__(256) void va_sort_hist_kernel(const u32 *__restrict
   const u32 n = min(*n_dev, cap);
   const u32 blk = rk_sort_block(blockIdx.x, nbs);
   const u32 base = blk * RK_SORT_B;
-#pragma unroll 4
+  // the thread's eight keys are loaded before the first LDS atomic, so their loads are in flight
+  // together
+  u32 k[RK_SORT_I];
+#pragma unroll
   for(u32 i = 0; i < RK_SORT_I; i++)
   {
     const u32 idx = base + i * RK_SORT_T + threadIdx.x;
-    if(idx < n)
-      atomicAdd(&s_h[(keys[idx] >> shift) & 255u], 1u);
+    k[i] = idx < n ? keys[idx] : 0xffffffffu;
   }
+#pragma unroll
+  for(u32 i = 0; i < RK_SORT_I; i++)
+    if(base + i * RK_SORT_T + threadIdx.x < n)
+      atomicAdd(&s_h[(k[i] >> shift) & 255u], 1u);
   __syncthreads();
   hist[threadIdx.x * nbs + blk] = s_h[threadIdx.x];
 }
@@ -241,6 +247,17 @@ __global__ __launch_bounds__(256) void va_sort_scatter_kernel(const u32 *__restr
   const u32 base = blk * RK_SORT_B;
   if(base >= n)
     return;
+  // The wave's 512 pairs are loaded first, all sixteen loads per lane in flight together: the
+  // wave syncs of the ranking loop below order memory, so loads inside it would be issued one
+  // step at a time, each step waiting out a full memory latency.
+  u32 key[RK_SORT_I], val[RK_SORT_I], rank[RK_SORT_I];
+#pragma unroll
+  for(u32 j = 0; j < RK_SORT_I; j++)
+  {
+    const u32 idx = base + w * (RK_SORT_I * 64u) + j * 64u + lane;
+    key[j] = idx < n ? kin[idx] : 0u;
+    val[j] = idx < n ? vin[idx] : 0u;
+  }
   // this block's first slot for digit tid: the pairs of lower digits (an exclusive scan of the
   // digit totals) + those of digit tid in earlier blocks (the row scan)
   {
@@ -252,15 +269,12 @@ __global__ __launch_bounds__(256) void va_sort_scatter_kernel(const u32 *__restr
     s_cnt[k][tid] = 0u;
   __syncthreads();
   const u64 below = lane == 0u ? 0ull : (~0ull >> (64u - lane));
-  u32 key[RK_SORT_I], val[RK_SORT_I], rank[RK_SORT_I];
 #pragma unroll
   for(u32 j = 0; j < RK_SORT_I; j++)
   {
     const u32 idx = base + w * (RK_SORT_I * 64u) + j * 64u + lane;
     const bool valid = idx < n;
-    const u32 k = valid ? kin[idx] : 0u;
-    const u32 v = valid ? vin[idx] : 0u;
-    const u32 d = (k >> shift) & 255u;
+    const u32 d = (key[j] >> shift) & 255u;
     u64 peers = __ballot(valid);
 #pragma unroll
     for(u32 b = 0; b < 8u; b++)
@@ -274,8 +288,6 @@ __global__ __launch_bounds__(256) void va_sort_scatter_kernel(const u32 *__restr
     if(valid && r == 0u)
       s_cnt[w][d] = cur + (u32)__popcll(peers);
     rk_wave_sync();
-    key[j] = k;
-    val[j] = v;
     rank[j] = valid ? cur + r : 0xffffffffu;
   }
   __syncthreads();
