@@ -539,6 +539,17 @@ __global__ __launch_bounds__(256) void va_finish_kernel(uint64_t *__restrict__ r
 {
   __shared__ uint64_t s_v[2 * kFinishMaxRows * 4];
   const uint64_t t_start = (uint64_t)wall_clock64();
+  // the words that do not depend on the sums are loaded together with the rows
+  uint64_t w0 = 0, w10 = 0, w11 = 0, w12 = 0;
+  uint32_t n = 0;
+  if(threadIdx.x == 0u)
+  {
+    w0 = *pair_total;
+    w10 = stamps[0];
+    w11 = stamps[1];
+    w12 = stamps[2];
+    n = *seq + 1u;  // one workgroup per launch, launches in stream order
+  }
   const uint32_t nw = 2u * nrows * 4u;
   for(uint32_t i = threadIdx.x; i < nw; i += 256u)
   {
@@ -556,18 +567,20 @@ __global__ __launch_bounds__(256) void va_finish_kernel(uint64_t *__restrict__ r
   }
   if(threadIdx.x == 0u)
   {
-    out[0] = *pair_total;
-    out[10] = stamps[0];
-    out[11] = stamps[1];
-    out[12] = stamps[2];
+    out[0] = w0;
+    out[10] = w10;
+    out[11] = w11;
+    out[12] = w12;
     out[13] = t_start;
+    *seq = n;
   }
-  __threadfence_system();  // each writer's words of out, before the sequence number
+  // Every word of out is written by the first wave: only its lanes fence their words before the
+  // sequence number (a system-scope fence also writes back the L2, once per wave that runs it).
+  if(threadIdx.x < 64u)
+    __threadfence_system();
   __syncthreads();
   if(threadIdx.x == 0u)
   {
-    const uint32_t n = *seq + 1u;  // one workgroup per launch, launches in stream order
-    *seq = n;
     __threadfence_system();
     *(volatile uint64_t *)&out[14] = n;
   }
