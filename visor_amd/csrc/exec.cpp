@@ -1116,8 +1116,10 @@ VkResult Queue::RunPass(const std::vector<Op *> &passes, bool fused_clear, ExecS
             return hipErrorUnknown;
           uint32_t *st_ = (uint32_t *)d_start;
           void *rargs[] = {&sorted_keys, &total_dev, &cap, &tile_total, &st_};
-          // a grid-stride pass over the (at most cap + 1) bin boundaries
-          const uint32_t rblocks = (uint32_t)std::min<uint64_t>(((uint64_t)cap + 256) / 256, 1024);
+          // a grid-stride pass over the (at most cap + 1) bin boundaries, one per thread up to 4M
+          // pairs (a thread that strides runs its key loads one step after another); workgroups
+          // past the pass's pair count leave at once
+          const uint32_t rblocks = (uint32_t)std::min<uint64_t>(((uint64_t)cap + 256) / 256, 16384);
           VA_TRY(hipModuleLaunchKernel(mod->ranges, rblocks, 1, 1, 256, 1, 1, 0, stream, rargs, nullptr));
         }
         else
