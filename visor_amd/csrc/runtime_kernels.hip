@@ -16,16 +16,21 @@ typedef uint64_t u64;
 // ---------------------------------------------------------------------------------------------
 // Wave and block scans (64-lane wavefronts)
 
+// Inclusive prefix sum over the wave's 64 lanes (all active) with DPP moves only, no LDS-routed
+// shuffles: a scan inside each row of 16 lanes (row_shr by 1, 2, 4, 8; a lane whose source lies
+// before its row's start adds 0), then row_bcast:15 adds lane 15 of rows 0 and 2 to rows 1 and 3,
+// and row_bcast:31 adds lane 31 to rows 2 and 3.
 __device__ __forceinline__ u32 rk_wave_inclusive(u32 v, u32 lane)
 {
-#pragma unroll
-  for(u32 o = 1u; o < 64u; o <<= 1)
-  {
-    const u32 t = (u32)__shfl_up((int)v, o);
-    if(lane >= o)
-      v += t;
-  }
-  return v;
+  (void)lane;
+  u32 x = v;
+  x += (u32)__builtin_amdgcn_update_dpp(0, (int)x, 0x111, 0xf, 0xf, false);  // row_shr:1
+  x += (u32)__builtin_amdgcn_update_dpp(0, (int)x, 0x112, 0xf, 0xf, false);  // row_shr:2
+  x += (u32)__builtin_amdgcn_update_dpp(0, (int)x, 0x114, 0xf, 0xf, false);  // row_shr:4
+  x += (u32)__builtin_amdgcn_update_dpp(0, (int)x, 0x118, 0xf, 0xf, false);  // row_shr:8
+  x += (u32)__builtin_amdgcn_update_dpp(0, (int)x, 0x142, 0xa, 0xf, false);  // row_bcast:15
+  x += (u32)__builtin_amdgcn_update_dpp(0, (int)x, 0x143, 0xc, 0xf, false);  // row_bcast:31
+  return x;
 }
 
 // Exclusive scan over the block (blockDim.x = 64 * NW threads); s_w holds NW words.
